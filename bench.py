@@ -1,0 +1,185 @@
+"""Benchmark: Mrays/s of the persistent batched wavefront on the material-divergent scene.
+
+Workload: sphere_grid (1024 spheres on a jittered 32x32 grid, Lambert / metal (fuzz) / dielectric,
+sky-gradient emitter used for next-event estimation) at 1280x720, 64 spp, depth 8, seed 1234, FP32,
+synthetic procedural input. One step = one full render of the workload (reset + all 64 passes),
+scene and film resident in HBM; the download of the image is outside the timed region.
+
+Prints one JSON line (the driver's contract) with:
+  value        (extension + shadow rays of all K steps) / wall time of the K steps, in Mrays/s
+  p50/p99      per-step render latency (HIP events around each step)
+  roofline     dominant kernel of the flagship arm: algorithmic bytes per launch / average launch
+               duration (stage-timed render with HIP events on the engine's stream), against 8 TB/s
+  cpu_baseline the oracle (a C restatement of the same estimator) on the host cores, bounded sample
+  quality      persistent == megakernel bitwise, zero overflows / invalid hits / non-finite pixels
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "graphics-book-tutorials_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+METRIC = "Mrays/s at 1280x720x64spp depth 8 per scene; p50/p99 render latency"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per item of each persistent-engine kernel (device layout of
+# graphics-book-tutorials_amd/csrc/wf_pathpool.h and wf_persistent.hip):
+#   extend_classify per live path: active index 4, ray + throughput + radiance 64, radiance 16,
+#                   surface 32, alive flag 4, queue entry 4 (the BVH stays in LDS / L2)
+#   shade per queue entry: entry 4, surface 32, ray + throughput 48, key 8, write ray + throughput 48, flag 4
+#   shadow per shadow ray: entry 48, radiance read-modify-write 32
+#   retire per live path: index 4 + flag 4 (reduce), index 4 + flag 4 + work id 8, then 4 (survivor)
+#                   or 4 + radiance 16 + sample 16 (finished)
+BYTES_PER_ITEM = {"extend_classify": 124, "shade": 144, "shadow": 80, "retire": 36}
+
+
+def percentile(v, p):
+    v = sorted(v)
+    if len(v) == 1:
+        return v[0]
+    x = p * (len(v) - 1)
+    i = int(x)
+    f = x - i
+    return v[i] + (v[min(i + 1, len(v) - 1)] - v[i]) * f
+
+
+def cpu_baseline(wfamd, scene_name, w, h, spp_sample):
+    import numpy as np
+    import pyoracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    s = wfamd.Scene(scene_name, w, h)
+    o = pyoracle.OScene(s.arrays())
+    t0 = time.perf_counter()
+    _, cnt = o.render_sums(0, spp_sample, 1234, 8, threads=threads)
+    dt = time.perf_counter() - t0
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": (cnt[0] + cnt[1]) / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{scene_name} {w}x{h}, {spp_sample} spp (of 64), depth 8, seed 1234; {cnt[0]} extension + "
+                      f"{cnt[1]} shadow rays in {dt:.2f} s; {model}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scene", default="sphere_grid")
+    ap.add_argument("--width", type=int, default=1280)
+    ap.add_argument("--height", type=int, default=720)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (the path shards by pixel tiles; see DESIGN.md); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import wfamd
+
+    wfamd.lib()
+    dev = wfamd.query_device()
+    print(f"[bench] device {dev['name']} {dev['arch']} {dev['compute_units']} CUs", file=sys.stderr, flush=True)
+    scene = wfamd.Scene(args.scene, args.width, args.height)
+    spp, depth, seed = args.spp, 8, 1234
+
+    tuning = wfamd.params(spp=spp, seed=seed, max_depth=depth)
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tuned = wfamd.autotune([(scene, spp, seed, depth)], probe_spp=4)
+        tuning.block_size, tuning.pool_paths, tuning.scan_tile = tuned.block_size, tuned.pool_paths, tuned.scan_tile
+        print(f"[bench] autotune: block {tuned.block_size} pool {tuned.pool_paths} tile {tuned.scan_tile} "
+              f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+
+    r = wfamd.Renderer(scene, "persistent")
+    for _ in range(args.warmup):
+        r.reset()
+        r.run_passes(0, spp, tuning)
+    lat, ext, sh = [], 0, 0
+    last = None
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.reset()
+        st = r.run_passes(0, spp, tuning)
+        lat.append(st.render_ms)
+        ext += st.extension_rays
+        sh += st.shadow_rays
+        last = st
+    elapsed = time.perf_counter() - t0
+    print(f"[bench] persistent: {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    sums_p = r.download_sums()
+    rgb_p, var_p = wfamd.finalize(*sums_p, spp)
+
+    # Megakernel on the same workload: throughput beside the flagship, and the bitwise gate.
+    mk = wfamd.Renderer(scene, "megakernel")
+    mk.reset()
+    mk.run_passes(0, spp, tuning)
+    mk_lat = []
+    for _ in range(max(2, min(5, args.steps))):
+        mk.reset()
+        mst = mk.run_passes(0, spp, tuning)
+        mk_lat.append(mst.render_ms)
+    sums_m = mk.download_sums()
+    mk.close()
+    bitwise = all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(sums_p, sums_m))
+    rays_per_render = last.extension_rays + last.shadow_rays
+    mk_mrays = (mst.extension_rays + mst.shadow_rays) / (percentile(mk_lat, 0.5) * 1e3)
+    print(f"[bench] megakernel median {percentile(mk_lat, 0.5):.2f} ms ({mk_mrays:.0f} Mrays/s); bitwise={bitwise}",
+          file=sys.stderr, flush=True)
+
+    # Stage-timed render: per-kernel average launch durations and item counts.
+    roof = None
+    try:
+        stages = r.stage_profile(0, spp, tuning) if hasattr(r, "stage_profile") else None
+    except Exception as e:   # noqa: BLE001
+        print(f"[bench] stage profile failed: {e}", file=sys.stderr)
+        stages = None
+    if stages:
+        dom = max(stages, key=lambda k: stages[k]["ms"])
+        d = stages[dom]
+        bytes_per_launch = BYTES_PER_ITEM[dom] * d["items"] / max(d["launches"], 1)
+        avg_s = d["ms"] / 1e3 / max(d["launches"], 1)
+        achieved = bytes_per_launch / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "avg_launch_us": avg_s * 1e6, "bytes_per_item": BYTES_PER_ITEM[dom],
+                "items_per_launch": d["items"] / max(d["launches"], 1),
+                "stages_ms": {k: v["ms"] for k, v in stages.items()}}
+    r.close()
+
+    cpu = None if args.no_cpu else cpu_baseline(wfamd, args.scene, args.width, args.height, 4)
+    value = (ext + sh) / elapsed / 1e6
+    out = {
+        "metric": METRIC, "value": value, "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic: procedural sphere_grid scene (1024 jittered spheres, seeded), no external data",
+        "config": {"workload": f"{args.scene} {args.width}x{args.height} {spp}spp depth {depth} seed {seed} "
+                               f"(material-divergent scene)", "arm": "persistent batched wavefront",
+                   "block_size": tuning.block_size, "pool_paths": tuning.pool_paths, "scan_tile": tuning.scan_tile,
+                   "parallelism": "single GPU"},
+        "p50_ms": percentile(lat, 0.5), "p99_ms": percentile(lat, 0.99),
+        "extension_mrays_per_s": ext / elapsed / 1e6, "rays_per_render": rays_per_render,
+        "megakernel": {"mrays_per_s": mk_mrays, "p50_ms": percentile(mk_lat, 0.5)},
+        "speedup_vs_megakernel": value / mk_mrays if mk_mrays else None,
+        "quality": {"persistent_equals_megakernel_bitwise": bool(bitwise),
+                    "queue_overflows": int(last.queue_overflows), "invalid_hits": int(last.invalid_hits),
+                    "nonfinite_pixels": int((~np.isfinite(rgb_p.reshape(-1, 3))).any(axis=1).sum()),
+                    "nonfinite_var": int((~np.isfinite(var_p)).sum())},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

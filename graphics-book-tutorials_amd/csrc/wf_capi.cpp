@@ -1,0 +1,857 @@
+// C ABI over the C++ engine (include/wfamd.h). Every entry point catches exceptions and maps
+// them to status codes with a per-thread message, so no C++ exception crosses the boundary.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/wfamd.h"
+#include "wf_cpu.h"
+#include "wf_persistent.h"
+#include "wf_queues.h"
+#include "wf_renderer.h"
+#include "wf_handles.h"
+#include "wf_scene.h"
+
+struct wf_renderer {
+    std::unique_ptr<wf::GpuRenderer> r;
+    std::string name;
+};
+struct wf_scan_ws {
+    wf::gpu::ScanWorkspace ws;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) { return wf::set_last_error(code, msg); }
+
+template <class F>
+int guard(F&& f) {
+    try {
+        return f();
+    } catch (const std::invalid_argument& e) {
+        return fail(WF_ERR_INVALID, e.what());
+    } catch (const std::exception& e) {
+        return fail(WF_ERR_RUNTIME, e.what());
+    } catch (...) {
+        return fail(WF_ERR_RUNTIME, "unknown error");
+    }
+}
+
+wf::RenderParams to_params(const wf_render_params* p) {
+    wf::RenderParams r;
+    if (!p) return r;
+    r.spp = p->spp;
+    r.seed = p->seed;
+    r.max_depth = p->max_depth;
+    r.threads = p->threads;
+    r.count_rays = p->count_rays != 0;
+    r.stage_timing = p->stage_timing != 0;
+    r.queue_capacity = p->queue_capacity;
+    r.trace_bounces = p->trace_bounces != 0;
+    r.block_size = p->block_size;
+    r.pool_paths = p->pool_paths;
+    r.scan_tile = p->scan_tile;
+    r.lds_nodes = p->lds_nodes;
+    return r;
+}
+
+void check_params(const wf::RenderParams& p) {
+    if (p.spp < 1) throw std::invalid_argument("spp must be >= 1");
+    if (p.max_depth < 0 || p.max_depth > 1024) throw std::invalid_argument("max_depth out of range");
+}
+
+void to_stats(const wf::RenderStats& s, wf_render_stats* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->render_ms = s.render_ms;
+    o->total_ms = s.total_ms;
+    o->extension_rays = s.extension_rays;
+    o->shadow_rays = s.shadow_rays;
+    for (int k = 0; k < 5; ++k) o->queue_high_water[k] = s.queue_high_water[k];
+    o->queue_overflows = s.queue_overflows;
+    o->invalid_hits = s.invalid_hits;
+    o->peak_vram_bytes = s.peak_vram_bytes;
+    o->stage_ms[0] = s.stages.extend;
+    o->stage_ms[1] = s.stages.classify;
+    o->stage_ms[2] = s.stages.shade_lambert;
+    o->stage_ms[3] = s.stages.shade_metal;
+    o->stage_ms[4] = s.stages.shade_dielectric;
+    o->stage_ms[5] = s.stages.shadow;
+    o->stage_ms[6] = s.stages.compact;
+    o->stage_ms[7] = s.stages.other;
+    o->iterations = s.iterations;
+}
+
+float bits_f(int v) {
+    float f;
+    std::memcpy(&f, &v, 4);
+    return f;
+}
+int bits_i(float f) {
+    int v;
+    std::memcpy(&v, &f, 4);
+    return v;
+}
+
+// Uploads each distinct host scene once; the views stay valid while the cache lives.
+struct SceneCache {
+    std::map<const wf_scene*, std::unique_ptr<wf::gpu::DeviceScene>> map;
+    const wf::SceneView* get(const wf_scene* s) {
+        auto it = map.find(s);
+        if (it == map.end()) {
+            auto d = std::make_unique<wf::gpu::DeviceScene>();
+            d->upload(s->s);
+            it = map.emplace(s, std::move(d)).first;
+        }
+        return &it->second->view;
+    }
+    uint64_t bytes() const {
+        uint64_t b = 0;
+        for (auto& kv : map) b += kv.second->bytes;
+        return b;
+    }
+};
+
+struct DevFilmAlloc {
+    wf::gpu::DeviceFilm film{};
+    int pixels = 0;
+    void alloc(int px, hipStream_t stream) {
+        pixels = px;
+        wf::gpu::check(hipMalloc(&film.rgb, (size_t)px * 5 * sizeof(float)), "hipMalloc film");
+        film.lum = film.rgb + 3 * (size_t)px;
+        film.lum2 = film.lum + px;
+        film.pixels = px;
+        wf::gpu::check(hipMemsetAsync(film.rgb, 0, (size_t)px * 5 * sizeof(float), stream), "hipMemset film");
+    }
+    void download(int spp, float* rgb, float* lum_var, hipStream_t stream) {
+        std::vector<float> h((size_t)pixels * 5);
+        wf::gpu::check(hipMemcpyAsync(h.data(), film.rgb, h.size() * sizeof(float), hipMemcpyDeviceToHost, stream),
+                       "download film");
+        wf::gpu::check(hipStreamSynchronize(stream), "download sync");
+        wf::finalize_film(h.data(), h.data() + 3 * (size_t)pixels, h.data() + 4 * (size_t)pixels, pixels, spp, rgb,
+                          lum_var);
+    }
+    void release() {
+        if (film.rgb) (void)hipFree(film.rgb);
+        film = wf::gpu::DeviceFilm{};
+    }
+};
+
+}  // namespace
+
+int wf::set_last_error(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+extern "C" {
+
+const char* wf_last_error(void) { return g_err.c_str(); }
+const char* wf_version(void) { return "wfamd 0.1 (gfx950)"; }
+
+// ---------------------------------------------------------------------------------------------
+// scenes
+// ---------------------------------------------------------------------------------------------
+int wf_scene_create(const char* name, int width, int height, wf_scene** out) {
+    return guard([&] {
+        if (!name || !out) throw std::invalid_argument("null argument");
+        if (width < 1 || height < 1 || width > 8192 || height > 8192) throw std::invalid_argument("image size outside 1..8192");
+        auto s = std::make_unique<wf_scene>();
+        s->s = wf::make_scene(name, width, height);
+        *out = s.release();
+        return (int)WF_OK;
+    });
+}
+
+int wf_scene_from_arrays(const wf_scene_desc* d, wf_scene** out) {
+    return guard([&] {
+        if (!d || !out) throw std::invalid_argument("null argument");
+        if (d->width < 1 || d->height < 1 || d->width > 8192 || d->height > 8192) throw std::invalid_argument("image size outside 1..8192");
+        auto s = std::make_unique<wf_scene>();
+        wf::Scene& sc = s->s;
+        sc.name = "arrays";
+        sc.width = d->width;
+        sc.height = d->height;
+        for (int i = 0; i < d->n_spheres; ++i) {
+            const float* p = d->spheres + 8 * i;
+            wf::Sphere sp{};
+            sp.center = wf::Vec3{p[0], p[1], p[2]};
+            sp.radius = p[3];
+            sp.material = bits_i(p[4]);
+            sc.spheres.push_back(sp);
+        }
+        for (int i = 0; i < d->n_vertices; ++i)
+            sc.vertices.push_back(wf::Vec3{d->vertices[3 * i], d->vertices[3 * i + 1], d->vertices[3 * i + 2]});
+        for (int i = 0; i < d->n_triangles; ++i) {
+            const int* t = d->triangles + 4 * i;
+            sc.triangles.push_back(wf::Triangle{t[0], t[1], t[2], t[3]});
+        }
+        for (int i = 0; i < d->n_materials; ++i) {
+            const float* m = d->materials + 12 * i;
+            wf::Material mt{};
+            mt.type = bits_i(m[0]);
+            mt.ior = m[1];
+            mt.fuzz = m[2];
+            mt.albedo = wf::Vec3{m[4], m[5], m[6]};
+            mt.emission = wf::Vec3{m[8], m[9], m[10]};
+            if (mt.type < 0 || mt.type > 3) throw std::invalid_argument("material type out of range");
+            sc.materials.push_back(mt);
+        }
+        if (d->env) {
+            const float* e = d->env;
+            sc.env.env = wf::Vec3{e[0], e[1], e[2]};
+            sc.env.sky_mode = bits_i(e[3]);
+            sc.env.sky_lo = wf::Vec3{e[4], e[5], e[6]};
+            sc.env.sky_light = bits_i(e[7]);
+            sc.env.sky_hi = wf::Vec3{e[8], e[9], e[10]};
+            sc.env.sun_light = bits_i(e[11]);
+            sc.env.sun_dir = wf::Vec3{e[12], e[13], e[14]};
+            sc.env.sun_E = wf::Vec3{e[16], e[17], e[18]};
+        }
+        if (d->camera) {
+            const float* c = d->camera;
+            sc.camera.position = wf::Vec3{c[0], c[1], c[2]};
+            sc.camera.look_at = wf::Vec3{c[3], c[4], c[5]};
+            sc.camera.up = wf::Vec3{c[6], c[7], c[8]};
+            sc.camera.vfov_deg = c[9];
+        }
+        try {
+            wf::finalize_scene(sc);
+        } catch (const std::runtime_error& e) {
+            throw std::invalid_argument(e.what());
+        }
+        *out = s.release();
+        return (int)WF_OK;
+    });
+}
+
+int wf_scene_orbit(wf_scene* scene, float yaw_deg) {
+    return guard([&] {
+        if (!scene) throw std::invalid_argument("null scene");
+        wf::orbit_camera(scene->s, yaw_deg);
+        return (int)WF_OK;
+    });
+}
+
+void wf_scene_destroy(wf_scene* scene) { delete scene; }
+
+int wf_scene_info_get(const wf_scene* scene, wf_scene_info* out) {
+    return guard([&] {
+        if (!scene || !out) throw std::invalid_argument("null argument");
+        const wf::Scene& s = scene->s;
+        std::memset(out, 0, sizeof(*out));
+        out->width = s.width;
+        out->height = s.height;
+        out->n_spheres = (int)s.spheres.size();
+        out->n_triangles = (int)s.triangles.size();
+        out->n_vertices = (int)s.vertices.size();
+        out->n_materials = (int)s.materials.size();
+        out->n_lights = (int)s.lights.size();
+        out->n_nodes = (int)s.nodes.size();
+        wf::BvhStats st = wf::bvh_stats(s);
+        out->bvh_leaves = st.leaves;
+        out->bvh_internal = st.internal;
+        out->bvh_max_depth = st.max_depth;
+        out->bvh_sah_cost = st.sah_cost;
+        out->hash = wf::scene_hash(s);
+        return (int)WF_OK;
+    });
+}
+
+int wf_scene_export(const wf_scene* scene, float* spheres, float* vertices, int* triangles, float* materials, float* env,
+                    float* camera) {
+    return guard([&] {
+        if (!scene) throw std::invalid_argument("null scene");
+        const wf::Scene& s = scene->s;
+        if (spheres)
+            for (size_t i = 0; i < s.spheres.size(); ++i) {
+                float* p = spheres + 8 * i;
+                const wf::Sphere& sp = s.spheres[i];
+                p[0] = sp.center.x; p[1] = sp.center.y; p[2] = sp.center.z; p[3] = sp.radius;
+                p[4] = bits_f(sp.material); p[5] = p[6] = p[7] = 0.0f;
+            }
+        if (vertices)
+            for (size_t i = 0; i < s.vertices.size(); ++i) {
+                vertices[3 * i] = s.vertices[i].x;
+                vertices[3 * i + 1] = s.vertices[i].y;
+                vertices[3 * i + 2] = s.vertices[i].z;
+            }
+        if (triangles)
+            for (size_t i = 0; i < s.triangles.size(); ++i) {
+                const wf::Triangle& t = s.triangles[i];
+                triangles[4 * i] = t.i0; triangles[4 * i + 1] = t.i1; triangles[4 * i + 2] = t.i2; triangles[4 * i + 3] = t.material;
+            }
+        if (materials)
+            for (size_t i = 0; i < s.materials.size(); ++i) {
+                const wf::Material& m = s.materials[i];
+                float* p = materials + 12 * i;
+                p[0] = bits_f(m.type); p[1] = m.ior; p[2] = m.fuzz; p[3] = 0.0f;
+                p[4] = m.albedo.x; p[5] = m.albedo.y; p[6] = m.albedo.z; p[7] = 0.0f;
+                p[8] = m.emission.x; p[9] = m.emission.y; p[10] = m.emission.z; p[11] = 0.0f;
+            }
+        if (env) {
+            const wf::EnvParams& e = s.env;
+            float v[20] = {e.env.x, e.env.y, e.env.z, bits_f(e.sky_mode), e.sky_lo.x, e.sky_lo.y, e.sky_lo.z,
+                           bits_f(e.sky_light), e.sky_hi.x, e.sky_hi.y, e.sky_hi.z, bits_f(e.sun_light), e.sun_dir.x,
+                           e.sun_dir.y, e.sun_dir.z, 0.0f, e.sun_E.x, e.sun_E.y, e.sun_E.z, 0.0f};
+            std::memcpy(env, v, sizeof(v));
+        }
+        if (camera) {
+            const wf::Camera& c = s.camera;
+            float v[10] = {c.position.x, c.position.y, c.position.z, c.look_at.x, c.look_at.y, c.look_at.z,
+                           c.up.x, c.up.y, c.up.z, c.vfov_deg};
+            std::memcpy(camera, v, sizeof(v));
+        }
+        return (int)WF_OK;
+    });
+}
+
+int wf_scene_bvh(const wf_scene* scene, int* nodes, int* prim_order) {
+    return guard([&] {
+        if (!scene) throw std::invalid_argument("null scene");
+        const wf::Scene& s = scene->s;
+        if (nodes)
+            for (size_t i = 0; i < s.nodes.size(); ++i) {
+                const wf::BvhNode& n = s.nodes[i];
+                int* p = nodes + 12 * i;
+                p[0] = bits_i(n.lo.x); p[1] = bits_i(n.lo.y); p[2] = bits_i(n.lo.z); p[3] = n.skip;
+                p[4] = bits_i(n.hi.x); p[5] = bits_i(n.hi.y); p[6] = bits_i(n.hi.z);
+                p[7] = n.prim_start; p[8] = n.prim_count; p[9] = p[10] = p[11] = 0;
+            }
+        if (prim_order) std::memcpy(prim_order, s.prim_order.data(), s.prim_order.size() * sizeof(int));
+        return (int)WF_OK;
+    });
+}
+
+int wf_scene_camera_rig(const wf_scene* scene, float* rig12) {
+    return guard([&] {
+        if (!scene || !rig12) throw std::invalid_argument("null argument");
+        wf::CameraRig r = wf::make_camera_rig(scene->s.camera, scene->s.width, scene->s.height);
+        float v[12] = {r.origin.x, r.origin.y, r.origin.z, r.lower_left.x, r.lower_left.y, r.lower_left.z,
+                       r.horizontal.x, r.horizontal.y, r.horizontal.z, r.vertical.x, r.vertical.y, r.vertical.z};
+        std::memcpy(rig12, v, sizeof(v));
+        return (int)WF_OK;
+    });
+}
+
+// ---------------------------------------------------------------------------------------------
+// renderer
+// ---------------------------------------------------------------------------------------------
+int wf_renderer_create(const wf_scene* scene, int mode, wf_renderer** out) {
+    return guard([&] {
+        if (!scene || !out) throw std::invalid_argument("null argument");
+        if (mode != WF_MODE_MEGAKERNEL && mode != WF_MODE_WAVEFRONT && mode != WF_MODE_PERSISTENT)
+            throw std::invalid_argument("renderer mode must be megakernel, wavefront or persistent");
+        auto r = std::make_unique<wf_renderer>();
+        r->r = std::make_unique<wf::GpuRenderer>(scene->s, (wf::GpuMode)mode);
+        *out = r.release();
+        return (int)WF_OK;
+    });
+}
+
+void wf_renderer_destroy(wf_renderer* r) { delete r; }
+
+int wf_renderer_render(wf_renderer* r, const wf_render_params* p, wf_render_stats* stats, float* rgb, float* lum_var) {
+    return guard([&] {
+        if (!r || !p) throw std::invalid_argument("null argument");
+        wf::RenderParams rp = to_params(p);
+        check_params(rp);
+        wf::RenderStats st{};
+        auto t0 = std::chrono::steady_clock::now();
+        wf::RenderResult res = r->r->render(rp, &st);
+        auto t1 = std::chrono::steady_clock::now();
+        st.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        to_stats(st, stats);
+        if (rgb) std::memcpy(rgb, res.rgb.data(), res.rgb.size() * sizeof(float));
+        if (lum_var) std::memcpy(lum_var, res.lum_var.data(), res.lum_var.size() * sizeof(float));
+        return (int)WF_OK;
+    });
+}
+
+int wf_renderer_reset(wf_renderer* r) {
+    return guard([&] {
+        if (!r) throw std::invalid_argument("null renderer");
+        r->r->reset();
+        return (int)WF_OK;
+    });
+}
+
+int wf_renderer_run_passes(wf_renderer* r, int first_sample, int count, const wf_render_params* p,
+                           wf_render_stats* stats) {
+    return guard([&] {
+        if (!r || !p) throw std::invalid_argument("null argument");
+        if (first_sample < 0 || count < 0) throw std::invalid_argument("negative sample range");
+        wf::RenderParams rp = to_params(p);
+        if (rp.max_depth < 0 || rp.max_depth > 1024) throw std::invalid_argument("max_depth out of range");
+        wf::RenderStats st{};
+        r->r->run_passes(first_sample, count, rp, &st);
+        to_stats(st, stats);
+        return (int)WF_OK;
+    });
+}
+
+int wf_renderer_download(wf_renderer* r, int spp_done, float* rgb, float* lum_var) {
+    return guard([&] {
+        if (!r) throw std::invalid_argument("null renderer");
+        if (spp_done < 1) throw std::invalid_argument("spp_done must be >= 1");
+        wf::RenderResult res = r->r->download(spp_done);
+        if (rgb) std::memcpy(rgb, res.rgb.data(), res.rgb.size() * sizeof(float));
+        if (lum_var) std::memcpy(lum_var, res.lum_var.data(), res.lum_var.size() * sizeof(float));
+        return (int)WF_OK;
+    });
+}
+
+int wf_renderer_download_mean(wf_renderer* r, int spp_done, float* rgb) {
+    return guard([&] {
+        if (!r || !rgb) throw std::invalid_argument("null argument");
+        if (spp_done < 1) throw std::invalid_argument("spp_done must be >= 1");
+        std::vector<float> v;
+        r->r->download_mean(spp_done, &v);
+        std::memcpy(rgb, v.data(), v.size() * sizeof(float));
+        return (int)WF_OK;
+    });
+}
+
+int wf_renderer_download_sums(wf_renderer* r, float* rgb, float* lum, float* lum2) {
+    return guard([&] {
+        if (!r) throw std::invalid_argument("null renderer");
+        std::vector<float> a, b, c;
+        r->r->download_sums(rgb ? &a : nullptr, lum ? &b : nullptr, lum2 ? &c : nullptr);
+        if (rgb) std::memcpy(rgb, a.data(), a.size() * sizeof(float));
+        if (lum) std::memcpy(lum, b.data(), b.size() * sizeof(float));
+        if (lum2) std::memcpy(lum2, c.data(), c.size() * sizeof(float));
+        return (int)WF_OK;
+    });
+}
+
+uint64_t wf_renderer_device_bytes(const wf_renderer* r) { return r ? r->r->device_bytes() : 0; }
+
+const char* wf_renderer_device_name(const wf_renderer* r) {
+    if (!r) return "";
+    return r->r->device_name().c_str();
+}
+
+int wf_renderer_trace(const wf_renderer* r, wf_bounce_trace* out, int capacity) {
+    if (!r) return -1;
+    const auto& t = r->r->trace();
+    const int n = (int)t.size();
+    for (int i = 0; i < n && i < capacity && out; ++i) {
+        out[i].active = t[i].active;
+        out[i].lambert = t[i].lambert;
+        out[i].metal = t[i].metal;
+        out[i].dielectric = t[i].dielectric;
+        out[i].shadow = t[i].shadow;
+        out[i].next_active = t[i].next_active;
+        out[i].list_hash = t[i].list_hash;
+    }
+    return n;
+}
+
+// ---------------------------------------------------------------------------------------------
+// CPU arm
+// ---------------------------------------------------------------------------------------------
+int wf_render_cpu(const wf_scene* scene, const wf_render_params* p, wf_render_stats* stats, float* rgb, float* lum_var) {
+    return guard([&] {
+        if (!scene || !p) throw std::invalid_argument("null argument");
+        wf::RenderParams rp = to_params(p);
+        check_params(rp);
+        wf::CpuCounts c;
+        double ms = 0.0;
+        auto t0 = std::chrono::steady_clock::now();
+        wf::RenderResult res = wf::render_cpu(scene->s, rp.spp, rp.seed, rp.max_depth, rp.threads, &c, &ms);
+        auto t1 = std::chrono::steady_clock::now();
+        wf::RenderStats st{};
+        st.render_ms = ms;
+        st.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        st.extension_rays = c.extension_rays;
+        st.shadow_rays = c.shadow_rays;
+        st.invalid_hits = (long long)c.invalid_hits;
+        to_stats(st, stats);
+        if (rgb) std::memcpy(rgb, res.rgb.data(), res.rgb.size() * sizeof(float));
+        if (lum_var) std::memcpy(lum_var, res.lum_var.data(), res.lum_var.size() * sizeof(float));
+        return (int)WF_OK;
+    });
+}
+
+int wf_render_cpu_sums(const wf_scene* scene, int first_sample, int count, const wf_render_params* p,
+                       wf_render_stats* stats, float* rgb, float* lum, float* lum2) {
+    return guard([&] {
+        if (!scene || !p || !rgb || !lum || !lum2) throw std::invalid_argument("null argument");
+        if (first_sample < 0 || count < 0) throw std::invalid_argument("negative sample range");
+        wf::RenderParams rp = to_params(p);
+        const wf::Scene& s = scene->s;
+        wf::FilmSums film;
+        film.width = s.width;
+        film.height = s.height;
+        const size_t P = (size_t)s.width * s.height;
+        film.rgb.assign(rgb, rgb + 3 * P);
+        film.lum.assign(lum, lum + P);
+        film.lum2.assign(lum2, lum2 + P);
+        wf::CpuCounts c;
+        auto t0 = std::chrono::steady_clock::now();
+        wf::cpu_render_passes(s, first_sample, count, rp.seed, rp.max_depth, rp.threads, &film, &c);
+        auto t1 = std::chrono::steady_clock::now();
+        std::memcpy(rgb, film.rgb.data(), 3 * P * sizeof(float));
+        std::memcpy(lum, film.lum.data(), P * sizeof(float));
+        std::memcpy(lum2, film.lum2.data(), P * sizeof(float));
+        wf::RenderStats st{};
+        st.render_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        st.extension_rays = c.extension_rays;
+        st.shadow_rays = c.shadow_rays;
+        st.invalid_hits = (long long)c.invalid_hits;
+        to_stats(st, stats);
+        return (int)WF_OK;
+    });
+}
+
+int wf_finalize(const float* rgb_sum, const float* lum_sum, const float* lum2_sum, int pixels, int spp, float* rgb,
+                float* lum_var) {
+    return guard([&] {
+        if (!rgb_sum || !lum_sum || !lum2_sum || pixels < 0 || spp < 1) throw std::invalid_argument("bad argument");
+        wf::finalize_film(rgb_sum, lum_sum, lum2_sum, pixels, spp, rgb, lum_var);
+        return (int)WF_OK;
+    });
+}
+
+int wf_query_device(wf_device_facts* out) {
+    return guard([&] {
+        if (!out) throw std::invalid_argument("null argument");
+        wf::DeviceFacts f = wf::query_device();
+        std::memset(out, 0, sizeof(*out));
+        std::strncpy(out->name, f.name.c_str(), sizeof(out->name) - 1);
+        std::strncpy(out->arch, f.arch.c_str(), sizeof(out->arch) - 1);
+        out->compute_units = f.compute_units;
+        out->runtime_version = std::stoi(f.runtime_version);
+        out->driver_version = std::stoi(f.driver_version);
+        out->total_memory_bytes = f.total_memory_bytes;
+        out->free_memory_bytes = f.free_memory_bytes;
+        return (int)WF_OK;
+    });
+}
+
+// ---------------------------------------------------------------------------------------------
+// primitives
+// ---------------------------------------------------------------------------------------------
+int wf_scan_workspace_create(int capacity, wf_scan_ws** out) {
+    return guard([&] {
+        if (!out || capacity < 0) throw std::invalid_argument("bad argument");
+        auto w = std::make_unique<wf_scan_ws>();
+        wf::gpu::check(wf::gpu::scan_workspace_create(&w->ws, capacity), "scan_workspace_create");
+        *out = w.release();
+        return (int)WF_OK;
+    });
+}
+
+void wf_scan_workspace_destroy(wf_scan_ws* ws) {
+    if (!ws) return;
+    wf::gpu::scan_workspace_destroy(&ws->ws);
+    delete ws;
+}
+
+int wf_scan_exclusive_u32(const uint32_t* d_in, uint32_t* d_out, int n, uint32_t* d_total, wf_scan_ws* ws, void* stream,
+                          int tile) {
+    if (!ws) return fail(WF_ERR_INVALID, "null workspace");
+    hipError_t e = wf::gpu::scan_exclusive_u32(d_in, d_out, n, d_total, ws->ws, (hipStream_t)stream,
+                                               tile > 0 ? tile : wf::gpu::kDefaultScanTile);
+    if (e == hipErrorInvalidValue) return fail(WF_ERR_INVALID, "scan: n out of range or bad tile");
+    if (e != hipSuccess) return fail(WF_ERR_RUNTIME, hipGetErrorString(e));
+    return WF_OK;
+}
+
+int wf_compact_u32(const uint32_t* d_values, const uint32_t* d_flags, uint32_t* d_out, uint32_t* d_count,
+                   uint32_t* d_scratch, int n, wf_scan_ws* ws, void* stream, int tile) {
+    (void)d_scratch;   // the reduce-then-scan compaction needs no per-element scratch
+    if (!ws) return fail(WF_ERR_INVALID, "null workspace");
+    hipError_t e = wf::gpu::compact_u32(d_values, d_flags, d_out, d_count, n, ws->ws, (hipStream_t)stream,
+                                        tile > 0 ? tile : wf::gpu::kDefaultScanTile);
+    if (e == hipErrorInvalidValue) return fail(WF_ERR_INVALID, "compact: n out of range or bad tile");
+    if (e != hipSuccess) return fail(WF_ERR_RUNTIME, hipGetErrorString(e));
+    return WF_OK;
+}
+
+int wf_test_queue_push(const uint32_t* d_flags, int n, uint32_t* d_items, uint32_t capacity, uint32_t* d_counter,
+                       uint32_t* d_overflow, void* stream) {
+    hipError_t e = wf::gpu::test_queue_push(d_flags, n, d_items, capacity, d_counter, d_overflow, (hipStream_t)stream);
+    if (e == hipErrorInvalidValue) return fail(WF_ERR_INVALID, "queue push: n < 0");
+    if (e != hipSuccess) return fail(WF_ERR_RUNTIME, hipGetErrorString(e));
+    return WF_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// batch
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+wf::gpu::PersistentTuning to_tuning(const wf_render_params* p) {
+    wf::gpu::PersistentTuning t;
+    if (p) {
+        t.block_size = p->block_size;
+        t.pool_paths = p->pool_paths;
+        t.scan_tile = p->scan_tile;
+        t.lds_nodes = p->lds_nodes;
+    }
+    return t;
+}
+
+struct BatchSetup {
+    hipStream_t stream = nullptr;
+    wf::gpu::DevCounters* ctr = nullptr;
+    SceneCache scenes;
+    std::vector<DevFilmAlloc> films;
+    std::vector<wf::gpu::PersistentJob> jobs;
+    ~BatchSetup() {
+        for (DevFilmAlloc& f : films) f.release();
+        if (ctr) (void)hipFree(ctr);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void build(const wf_job* in, int n) {
+        wf::gpu::check(hipStreamCreate(&stream), "hipStreamCreate");
+        wf::gpu::check(hipMalloc(&ctr, sizeof(wf::gpu::DevCounters)), "hipMalloc counters");
+        wf::gpu::check(hipMemsetAsync(ctr, 0, sizeof(wf::gpu::DevCounters), stream), "hipMemset counters");
+        films.resize(n);
+        for (int i = 0; i < n; ++i) {
+            if (!in[i].scene) throw std::invalid_argument("job without a scene");
+            if (in[i].spp < 1) throw std::invalid_argument("job spp must be >= 1");
+            if (in[i].max_depth < 0 || in[i].max_depth > 1024) throw std::invalid_argument("job max_depth out of range");
+            const wf::SceneView* v = scenes.get(in[i].scene);
+            films[i].alloc(v->width * v->height, stream);
+            wf::gpu::PersistentJob j;
+            j.scene = v;
+            j.film = films[i].film;
+            j.first_sample = 0;
+            j.spp = in[i].spp;
+            j.seed = in[i].seed;
+            j.max_depth = in[i].max_depth;
+            jobs.push_back(j);
+        }
+    }
+};
+
+}  // namespace
+
+extern "C" int wf_batch_render(const wf_job* jobs, int n_jobs, const wf_render_params* tuning, wf_batch_stats* stats) {
+    return guard([&] {
+        if (!jobs || n_jobs < 1) throw std::invalid_argument("no jobs");
+        BatchSetup b;
+        b.build(jobs, n_jobs);
+        wf::gpu::PersistentEngine eng;
+        wf::gpu::PersistentTuning t = to_tuning(tuning);
+        hipEvent_t e0, e1;
+        wf::gpu::check(hipEventCreate(&e0), "hipEventCreate");
+        wf::gpu::check(hipEventCreate(&e1), "hipEventCreate");
+        wf::gpu::PersistentResult pr;
+        wf::gpu::check(hipEventRecord(e0, b.stream), "hipEventRecord");
+        eng.render(b.jobs.data(), n_jobs, t, b.ctr, b.stream, &pr);
+        wf::gpu::check(hipEventRecord(e1, b.stream), "hipEventRecord");
+        wf::gpu::check(hipEventSynchronize(e1), "batch sync");
+        float ms = 0.0f;
+        wf::gpu::check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        for (int i = 0; i < n_jobs; ++i) b.films[i].download(jobs[i].spp, jobs[i].rgb, jobs[i].lum_var, b.stream);
+        if (stats) {
+            wf::gpu::DevCounters c{};
+            wf::gpu::check(hipMemcpy(&c, b.ctr, sizeof(c), hipMemcpyDeviceToHost), "copy counters");
+            std::memset(stats, 0, sizeof(*stats));
+            stats->render_ms = ms;
+            stats->extension_rays = c.extension_rays;
+            stats->shadow_rays = c.shadow_rays;
+            stats->invalid_hits = c.invalid_hits;
+            stats->queue_overflows = c.overflows;
+            stats->iterations = pr.iterations;
+            stats->graph_launches = pr.graph_launches;
+            stats->block_size = t.block_size;
+            stats->pool_paths = eng.pool_paths();
+            stats->scan_tile = t.scan_tile;
+            stats->device_bytes = eng.bytes() + b.scenes.bytes();
+        }
+        return (int)WF_OK;
+    });
+}
+
+extern "C" int wf_autotune(const wf_job* jobs, int n_jobs, int probe_spp, wf_render_params* tuning) {
+    return guard([&] {
+        if (!jobs || n_jobs < 1 || !tuning || probe_spp < 1) throw std::invalid_argument("bad argument");
+        BatchSetup b;
+        b.build(jobs, n_jobs);
+        wf::gpu::PersistentTuning best = wf::gpu::autotune_persistent(b.jobs.data(), n_jobs, b.stream, probe_spp, nullptr);
+        tuning->block_size = best.block_size;
+        tuning->pool_paths = best.pool_paths;
+        tuning->scan_tile = best.scan_tile;
+        return (int)WF_OK;
+    });
+}
+
+// ---------------------------------------------------------------------------------------------
+// serving loop
+// ---------------------------------------------------------------------------------------------
+struct wf_server {
+    struct Req {
+        int64_t id;
+        wf_job job;
+        int engine_job = -1;
+        DevFilmAlloc film;
+        uint64_t bytes = 0;
+        int state = 0;   // 0 queued, 1 admitted, 2 done
+    };
+    uint64_t budget = 0, in_use = 0;
+    wf::gpu::PersistentTuning tune;
+    hipStream_t stream = nullptr;
+    wf::gpu::DevCounters* ctr = nullptr;
+    SceneCache scenes;
+    wf::gpu::PersistentEngine eng;
+    std::deque<Req*> queue;
+    std::map<int64_t, std::unique_ptr<Req>> reqs;
+    std::vector<Req*> running;
+    int64_t next_id = 1, submitted = 0, admitted = 0, completed = 0, rejected = 0;
+    double busy_ms = 0.0;
+    int jobs_in_table = 0;
+
+    ~wf_server() {
+        for (auto& kv : reqs) kv.second->film.release();
+        if (ctr) (void)hipFree(ctr);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    uint64_t request_bytes(const wf_job& j) const {
+        const uint64_t px = (uint64_t)j.scene->s.width * j.scene->s.height;
+        return px * 5 * sizeof(float) + px * (uint64_t)j.spp * sizeof(float4);
+    }
+};
+
+extern "C" {
+
+int wf_server_create(uint64_t budget_bytes, const wf_render_params* tuning, wf_server** out) {
+    return guard([&] {
+        if (!out) throw std::invalid_argument("null argument");
+        auto s = std::make_unique<wf_server>();
+        s->tune = to_tuning(tuning);
+        wf::gpu::check(hipStreamCreate(&s->stream), "hipStreamCreate");
+        wf::gpu::check(hipMalloc(&s->ctr, sizeof(wf::gpu::DevCounters)), "hipMalloc counters");
+        wf::gpu::check(hipMemsetAsync(s->ctr, 0, sizeof(wf::gpu::DevCounters), s->stream), "hipMemset counters");
+        s->eng.open(s->tune, s->ctr, s->stream);
+        s->budget = budget_bytes;
+        s->in_use = s->eng.pool_bytes();
+        if (s->in_use > budget_bytes) throw std::invalid_argument("budget smaller than the path pool");
+        *out = s.release();
+        return (int)WF_OK;
+    });
+}
+
+void wf_server_destroy(wf_server* s) { delete s; }
+
+int wf_server_submit(wf_server* s, const wf_job* job, int64_t* id) {
+    return guard([&] {
+        if (!s || !job || !job->scene || !id) throw std::invalid_argument("null argument");
+        if (job->spp < 1 || job->max_depth < 0 || job->max_depth > 1024) throw std::invalid_argument("bad job");
+        s->submitted++;
+        const uint64_t need = s->request_bytes(*job) + s->scenes.bytes();
+        if (s->eng.pool_bytes() + need > s->budget) {
+            s->rejected++;
+            return fail(WF_ERR_BUDGET, "request can never fit the device-memory budget");
+        }
+        auto r = std::make_unique<wf_server::Req>();
+        r->id = s->next_id++;
+        r->job = *job;
+        r->bytes = s->request_bytes(*job);
+        *id = r->id;
+        s->queue.push_back(r.get());
+        s->reqs.emplace(r->id, std::move(r));
+        return (int)WF_OK;
+    });
+}
+
+int wf_server_step(wf_server* s, int launches) {
+    return guard([&] {
+        if (!s) throw std::invalid_argument("null server");
+        auto t0 = std::chrono::steady_clock::now();
+        // When nothing runs, recycle the job table.
+        if (s->running.empty() && s->jobs_in_table > 0) {
+            s->eng.reset_jobs();
+            s->jobs_in_table = 0;
+        }
+        // Admit queued requests in arrival order while they fit the budget.
+        while (!s->queue.empty() && s->jobs_in_table < 4000) {
+            wf_server::Req* r = s->queue.front();
+            if (s->in_use + r->bytes > s->budget) break;
+            s->queue.pop_front();
+            const uint64_t scene_before = s->scenes.bytes();
+            const wf::SceneView* v = s->scenes.get(r->job.scene);
+            s->in_use += s->scenes.bytes() - scene_before;   // scene uploads stay resident
+            r->film.alloc(v->width * v->height, s->stream);
+            wf::gpu::PersistentJob j;
+            j.scene = v;
+            j.film = r->film.film;
+            j.spp = r->job.spp;
+            j.seed = r->job.seed;
+            j.max_depth = r->job.max_depth;
+            r->engine_job = s->eng.admit(j);
+            s->jobs_in_table++;
+            r->state = 1;
+            s->in_use += r->bytes;
+            s->running.push_back(r);
+            s->admitted++;
+        }
+        if (!s->running.empty() && launches > 0) {
+            s->eng.run(launches);
+            wf::gpu::PersistentProgress pr = s->eng.progress();
+            std::vector<wf_server::Req*> still;
+            for (wf_server::Req* r : s->running) {
+                if (s->eng.job_complete(r->engine_job, pr)) {
+                    s->eng.finish_job(r->engine_job);
+                    r->film.download(r->job.spp, r->job.rgb, r->job.lum_var, s->stream);
+                    r->film.release();
+                    r->state = 2;
+                    s->in_use -= r->bytes;
+                    s->completed++;
+                } else {
+                    still.push_back(r);
+                }
+            }
+            s->running.swap(still);
+        }
+        s->busy_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return (int)WF_OK;
+    });
+}
+
+int wf_server_poll(wf_server* s, int64_t id) {
+    if (!s) return -1;
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return -1;
+    return it->second->state == 2 ? 1 : 0;
+}
+
+int wf_server_stats_get(wf_server* s, wf_server_stats* out) {
+    return guard([&] {
+        if (!s || !out) throw std::invalid_argument("null argument");
+        std::memset(out, 0, sizeof(*out));
+        out->submitted = s->submitted;
+        out->admitted = s->admitted;
+        out->completed = s->completed;
+        out->rejected = s->rejected;
+        out->in_flight = (int64_t)s->running.size();
+        wf::gpu::DevCounters c{};
+        wf::gpu::check(hipMemcpy(&c, s->ctr, sizeof(c), hipMemcpyDeviceToHost), "copy counters");
+        out->extension_rays = c.extension_rays;
+        out->shadow_rays = c.shadow_rays;
+        out->bytes_in_use = s->in_use;
+        out->budget_bytes = s->budget;
+        out->busy_ms = s->busy_ms;
+        return (int)WF_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,129 @@
+// Device-side building blocks shared by the megakernel, the wavefront engines and the queue
+// primitives: device scene upload, film, counters, wave64 helpers and the LDS node source.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "wf_core.h"
+#include "wf_scene.h"
+
+namespace wf::gpu {
+
+inline void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Owns the device copies of a finalized scene's device-layout arrays.
+struct DeviceScene {
+    SceneView view{};
+    uint64_t bytes = 0;
+    std::vector<void*> allocs;
+    void upload(const Scene& s);
+    void release();
+    ~DeviceScene() { release(); }
+};
+
+// Per-pixel float sums (rgb interleaved, then luminance and its square).
+struct DeviceFilm {
+    float* rgb = nullptr;
+    float* lum = nullptr;
+    float* lum2 = nullptr;
+    int pixels = 0;
+};
+
+// Counters shared by every engine (src/gpu/engine.cuh:35-41). high_water order:
+// lambert, metal, dielectric, shadow, active.
+struct DevCounters {
+    unsigned long long extension_rays;
+    unsigned long long shadow_rays;
+    unsigned int invalid_hits;
+    unsigned int overflows;
+    unsigned int high_water[5];
+    unsigned int pad[3];
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Rank of this lane among the set lanes of a wave64 ballot.
+__device__ __forceinline__ uint32_t mbcnt64(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Wave-aggregated queue push: the lanes with pred reserve consecutive slots with one atomic per
+// wave (ballot + popcount, rank by mbcnt), then store `value`. Called by every active lane of the
+// wave with the same control flow. A slot at or past `capacity` is dropped and counted in
+// *overflow (one atomic per wave). Returns the slot or 0xFFFFFFFF. Contract as in
+// src/gpu/queues.cuh:12-27, for 64-wide waves.
+__device__ __forceinline__ uint32_t wave_push(uint32_t* items, uint32_t* counter, uint32_t capacity,
+                                              uint32_t value, bool pred, uint32_t* overflow) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return 0xFFFFFFFFu;
+    const uint32_t n = (uint32_t)__popcll(m);
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if ((int)lane_id() == leader) {
+        base = atomicAdd(counter, n);
+        const uint32_t keep = base >= capacity ? 0u : (capacity - base < n ? capacity - base : n);
+        if (keep < n) atomicAdd(overflow, n - keep);
+    }
+    base = __shfl(base, leader);
+    if (!pred) return 0xFFFFFFFFu;
+    const uint32_t slot = base + mbcnt64(m);
+    if (slot >= capacity) return 0xFFFFFFFFu;
+    items[slot] = value;
+    return slot;
+}
+
+// Reserve `pred ? 1 : 0` slots per lane with one atomic per wave; returns the slot or ~0u.
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t* counter, bool pred) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return 0xFFFFFFFFu;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if ((int)lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    return pred ? base + mbcnt64(m) : 0xFFFFFFFFu;
+}
+
+// Adds per-lane counts to 64-bit device counters with one atomic per wave.
+__device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v) {
+    uint32_t s = wave_sum_u32(v);
+    if (lane_id() == 0 && s) atomicAdd(dst, (unsigned long long)s);
+}
+__device__ __forceinline__ void wave_add_u32(unsigned int* dst, uint32_t v) {
+    uint32_t s = wave_sum_u32(v);
+    if (lane_id() == 0 && s) atomicAdd(dst, s);
+}
+
+// Node source that serves the first n_lds nodes (the breadth-first top of the tree) from LDS.
+struct LdsNodes {
+    const Node* lds;
+    const Node* mem;
+    int n_lds;
+    __device__ __forceinline__ Node operator()(int n) const { return n < n_lds ? lds[n] : mem[n]; }
+};
+
+// Stages the first n nodes into LDS; the whole block must call it.
+__device__ __forceinline__ void stage_nodes(Node* lds, const Node* nodes, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) lds[i] = nodes[i];
+    __syncthreads();
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; map them so that each XCD (and so each
+// L2) works on one contiguous range of logical block ids.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+    const int x = bid & 7, k = bid >> 3;
+    const int per = nblocks >> 3, rem = nblocks & 7;
+    return x * per + (x < rem ? x : rem) + k;
+}
+
+}  // namespace wf::gpu

@@ -1,0 +1,245 @@
+// Estimator test hooks (include/wfamd.h): the same shared functions evaluated on the host and on
+// gfx950, so tests can show bit-for-bit agreement of the two compilations.
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#include "../../include/wfamd.h"
+#include "wf_gpu.h"
+#include "wf_handles.h"
+
+namespace wf::gpu {
+namespace {
+
+__global__ void sincos_kernel(const float* x, int n, float* s, float* c) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) sincos_f32(x[i], &s[i], &c[i]);
+}
+
+__device__ __host__ inline void grid_angle(uint32_t k, float* s, float* c) {
+    const float u = (float)k * (1.0f / 16777216.0f);
+    const float phi = 2.0f * kPi * u;
+    sincos_f32(phi, s, c);
+}
+
+// Per-block FNV-1a over the (s, c) bits of a contiguous range; blocks are combined on the host.
+__global__ void sincos_grid_kernel(uint32_t k0, uint32_t k1, uint32_t per_block, uint64_t* out) {
+    __shared__ uint32_t bits[2 * 256];
+    const uint32_t base = k0 + blockIdx.x * per_block;
+    uint64_t h = 14695981039346656037ULL;
+    for (uint32_t off = 0; off < per_block; off += 256) {
+        const uint32_t k = base + off + threadIdx.x;
+        float s = 0.0f, c = 0.0f;
+        const bool ok = k < k1 && off + threadIdx.x < per_block;
+        if (ok) grid_angle(k, &s, &c);
+        bits[2 * threadIdx.x] = ok ? (uint32_t)__float_as_int(s) : 0u;
+        bits[2 * threadIdx.x + 1] = ok ? (uint32_t)__float_as_int(c) : 0u;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (uint32_t j = 0; j < 256; ++j) {
+                const uint32_t kk = base + off + j;
+                if (kk >= k1 || off + j >= per_block) break;
+                for (int w = 0; w < 2; ++w) {
+                    const uint32_t v = bits[2 * j + w];
+                    for (int b = 0; b < 4; ++b) {
+                        h ^= (v >> (8 * b)) & 0xffu;
+                        h *= 1099511628211ULL;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = h;
+}
+
+__global__ void intersect_kernel(SceneView s, const float* rays, int n, int anyhit, float* t, int* prim) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* q = rays + 8 * i;
+    Ray r;
+    r.o = Vec3{q[0], q[1], q[2]};
+    r.tmin = q[3];
+    r.d = Vec3{q[4], q[5], q[6]};
+    r.tmax = q[7];
+    Hit h;
+    intersect(s, r, &h, anyhit != 0);
+    t[i] = h.t;
+    prim[i] = h.prim >= 0 ? s.prims[h.prim].id : -1;
+}
+
+__global__ void trace_kernel(SceneView s, const int* pxys, int n, uint32_t seed, int max_depth, float* L,
+                             uint32_t* rays) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    RayCount rc{0, 0, 0};
+    const Vec3 v = trace_path(s, pxys[3 * i], pxys[3 * i + 1], (uint32_t)pxys[3 * i + 2], seed, max_depth, &rc);
+    L[3 * i] = v.x;
+    L[3 * i + 1] = v.y;
+    L[3 * i + 2] = v.z;
+    rays[2 * i] = rc.extension;
+    rays[2 * i + 1] = rc.shadow;
+}
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    explicit DevBuf(size_t n) { check(hipMalloc(&p, (n ? n : 1) * sizeof(T)), "hipMalloc test buffer"); }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void put(const T* h, size_t n) { check(hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy"); }
+    void get(T* h, size_t n) { check(hipMemcpy(h, p, n * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy"); }
+};
+
+uint64_t fnv_words(uint64_t h, uint32_t v) {
+    for (int b = 0; b < 4; ++b) {
+        h ^= (v >> (8 * b)) & 0xffu;
+        h *= 1099511628211ULL;
+    }
+    return h;
+}
+
+}  // namespace
+}  // namespace wf::gpu
+
+namespace {
+template <class F>
+int hook(F&& f) {
+    try {
+        f();
+        return WF_OK;
+    } catch (const std::invalid_argument& e) {
+        return wf::set_last_error(WF_ERR_INVALID, e.what());
+    } catch (const std::exception& e) {
+        return wf::set_last_error(WF_ERR_RUNTIME, e.what());
+    }
+}
+}  // namespace
+
+using namespace wf;
+using namespace wf::gpu;
+
+extern "C" {
+
+int wf_test_sincos(const float* x, int n, float* s, float* c, int device) {
+    return hook([&] {
+        if (n < 0 || (n > 0 && (!x || !s || !c))) throw std::invalid_argument("bad argument");
+        if (!device) {
+            for (int i = 0; i < n; ++i) sincos_f32(x[i], &s[i], &c[i]);
+            return;
+        }
+        DevBuf<float> dx(n), ds(n), dc(n);
+        dx.put(x, n);
+        if (n) sincos_kernel<<<(n + 255) / 256, 256>>>(dx.p, n, ds.p, dc.p);
+        check(hipGetLastError(), "sincos kernel");
+        check(hipDeviceSynchronize(), "sincos sync");
+        ds.get(s, n);
+        dc.get(c, n);
+    });
+}
+
+int wf_test_sincos_grid_hash(uint32_t k0, uint32_t k1, int device, uint64_t* hash) {
+    return hook([&] {
+        if (!hash || k1 < k0 || k1 > (1u << 24)) throw std::invalid_argument("bad range");
+        const uint32_t per_block = 4096;
+        const uint32_t nb = (k1 - k0 + per_block - 1) / per_block;
+        std::vector<uint64_t> parts(nb);
+        if (device) {
+            DevBuf<uint64_t> d(nb);
+            if (nb) sincos_grid_kernel<<<nb, 256>>>(k0, k1, per_block, d.p);
+            check(hipGetLastError(), "sincos grid kernel");
+            check(hipDeviceSynchronize(), "sincos grid sync");
+            d.get(parts.data(), nb);
+        } else {
+            for (uint32_t b = 0; b < nb; ++b) {
+                uint64_t h = 14695981039346656037ULL;
+                for (uint32_t k = k0 + b * per_block; k < k1 && k < k0 + (b + 1) * per_block; ++k) {
+                    float s, c;
+                    grid_angle(k, &s, &c);
+                    uint32_t sb, cb;
+                    std::memcpy(&sb, &s, 4);
+                    std::memcpy(&cb, &c, 4);
+                    h = fnv_words(fnv_words(h, sb), cb);
+                }
+                parts[b] = h;
+            }
+        }
+        uint64_t h = 14695981039346656037ULL;
+        for (uint64_t p : parts) {
+            h = fnv_words(h, (uint32_t)p);
+            h = fnv_words(h, (uint32_t)(p >> 32));
+        }
+        *hash = h;
+    });
+}
+
+int wf_test_intersect(const wf_scene* scene, const float* rays, int n, int anyhit, float* t, int* prim, int device) {
+    return hook([&] {
+        if (!scene || n < 0 || (n > 0 && (!rays || !t || !prim))) throw std::invalid_argument("bad argument");
+        const Scene& sc = scene->s;
+        if (!device) {
+            const SceneView v = host_view(sc);
+            for (int i = 0; i < n; ++i) {
+                const float* q = rays + 8 * i;
+                Ray r;
+                r.o = Vec3{q[0], q[1], q[2]};
+                r.tmin = q[3];
+                r.d = Vec3{q[4], q[5], q[6]};
+                r.tmax = q[7];
+                Hit h;
+                intersect(v, r, &h, anyhit != 0);
+                t[i] = h.t;
+                prim[i] = h.prim >= 0 ? sc.dprims[h.prim].id : -1;
+            }
+            return;
+        }
+        DeviceScene ds;
+        ds.upload(sc);
+        DevBuf<float> dr((size_t)8 * n), dt(n);
+        DevBuf<int> dp(n);
+        dr.put(rays, (size_t)8 * n);
+        if (n) intersect_kernel<<<(n + 255) / 256, 256>>>(ds.view, dr.p, n, anyhit, dt.p, dp.p);
+        check(hipGetLastError(), "intersect kernel");
+        check(hipDeviceSynchronize(), "intersect sync");
+        dt.get(t, n);
+        dp.get(prim, n);
+    });
+}
+
+int wf_test_trace_paths(const wf_scene* scene, const int* pxys, int n, uint32_t seed, int max_depth, float* L,
+                        uint32_t* rays, int device) {
+    return hook([&] {
+        if (!scene || n < 0 || (n > 0 && (!pxys || !L || !rays))) throw std::invalid_argument("bad argument");
+        const Scene& sc = scene->s;
+        for (int i = 0; i < n; ++i)
+            if (pxys[3 * i] < 0 || pxys[3 * i] >= sc.width || pxys[3 * i + 1] < 0 || pxys[3 * i + 1] >= sc.height)
+                throw std::invalid_argument("pixel outside the image");
+        if (!device) {
+            const SceneView v = host_view(sc);
+            for (int i = 0; i < n; ++i) {
+                RayCount rc{0, 0, 0};
+                Vec3 r = trace_path(v, pxys[3 * i], pxys[3 * i + 1], (uint32_t)pxys[3 * i + 2], seed, max_depth, &rc);
+                L[3 * i] = r.x;
+                L[3 * i + 1] = r.y;
+                L[3 * i + 2] = r.z;
+                rays[2 * i] = rc.extension;
+                rays[2 * i + 1] = rc.shadow;
+            }
+            return;
+        }
+        DeviceScene ds;
+        ds.upload(sc);
+        DevBuf<int> dq((size_t)3 * n);
+        DevBuf<float> dl((size_t)3 * n);
+        DevBuf<uint32_t> dc((size_t)2 * n);
+        dq.put(pxys, (size_t)3 * n);
+        if (n) trace_kernel<<<(n + 255) / 256, 256>>>(ds.view, dq.p, n, seed, max_depth, dl.p, dc.p);
+        check(hipGetLastError(), "trace kernel");
+        check(hipDeviceSynchronize(), "trace sync");
+        dl.get(L, (size_t)3 * n);
+        dc.get(rays, (size_t)2 * n);
+    });
+}
+
+}  // extern "C"

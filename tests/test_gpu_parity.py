@@ -1,0 +1,383 @@
+"""GPU parity: every HIP arm against the CPU arm and the oracle, through the C ABI.
+
+The contract (bitwise, because every arm runs the same estimator in the same FP32 operation order
+with correctly rounded divide/sqrt and no contraction, and films samples in sample order):
+  megakernel == wavefront == persistent == CPU arm, film sums and ray counts;
+  wavefront per-bounce integer work (active, queue counts, shadow count, compacted list) == the
+  oracle's CPU wavefront restatement.
+Reference tests followed: tests/test_gpu_render.cu:77-130,215-318, tests/catch/wavefront.cu,
+tests/catch/wavefront_failure.cu.
+"""
+import numpy as np
+import pytest
+
+from conftest import assert_bitwise
+
+pytestmark = pytest.mark.gpu
+
+SMALL = [("cornell", 64, 36), ("divergence", 64, 36), ("trianglehv", 64, 36), ("furnace", 64, 36),
+         ("cornell555", 64, 36), ("sphere_grid", 64, 36), ("heightfield", 48, 27)]
+
+
+@pytest.fixture(scope="module")
+def scenes(wf):
+    cache = {}
+
+    def get(name, w, h):
+        key = (name, w, h)
+        if key not in cache:
+            print(f"[scene] {name} {w}x{h}", flush=True)
+            cache[key] = wf.Scene(name, w, h)
+        return cache[key]
+    return get
+
+
+def gpu_sums(wf, scene, mode, spp, seed=1234, depth=8, **kw):
+    r = wf.Renderer(scene, mode)
+    p = wf.params(spp=spp, seed=seed, max_depth=depth, **kw)
+    r.reset()
+    st = r.run_passes(0, spp, p)
+    sums = r.download_sums()
+    r.close()
+    return sums, st
+
+
+def test_device_is_gfx950(wf):
+    f = wf.query_device()
+    print(f, flush=True)
+    assert f["arch"].startswith("gfx950"), f
+    assert f["total_memory_bytes"] > 200 * 2**30
+
+
+def test_sincos_device_equals_host_on_all_2pow24_angles(wf):
+    # Every angle the estimator produces is 2*pi*u with u = k * 2^-24.
+    hd = wf.test_sincos_grid_hash(0, 1 << 24, device=True)
+    hh = wf.test_sincos_grid_hash(0, 1 << 24, device=False)
+    assert hd == hh
+
+
+def test_sincos_device_equals_host_on_edge_values(wf):
+    x = np.array([0.0, -0.0, 1e-30, 1e-45, 0.5, 1.0, 1.5707963, 1.5707964, 3.1415925, 3.1415927, 4.712389,
+                  6.2831850, 6.2831855, -1.0, -3.0, 100.0, 1000.5], np.float32)
+    sd, cd = wf.test_sincos(x, device=True)
+    sh, ch = wf.test_sincos(x, device=False)
+    assert_bitwise(sd, sh, "sin")
+    assert_bitwise(cd, ch, "cos")
+
+
+@pytest.mark.parametrize("name,w,h", SMALL)
+def test_intersect_device_equals_host(wf, scenes, name, w, h):
+    s = scenes(name, w, h)
+    a = s.arrays()
+    pts = a["vertices"] if len(a["vertices"]) else a["spheres"][:, :3]
+    c = pts.mean(axis=0)
+    rad = float(np.abs(pts - c).max()) + 1.0
+    rs = np.random.default_rng(7)
+    n = 4096
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = c + rs.uniform(-rad, rad, (n, 3))
+    d = rs.normal(size=(n, 3))
+    rays[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 3] = 1e-4
+    rays[:, 7] = 3.402823466e38
+    for anyhit in (False, True):
+        td, pd = wf.test_intersect(s, rays, anyhit, device=True)
+        th, ph = wf.test_intersect(s, rays, anyhit, device=False)
+        assert_bitwise(td, th, f"{name} t anyhit={anyhit}")
+        assert np.array_equal(pd, ph)
+
+
+@pytest.mark.parametrize("name,w,h", SMALL)
+def test_megakernel_equals_cpu_bitwise(wf, scenes, name, w, h):
+    s = scenes(name, w, h)
+    spp = 4
+    (grgb, glum, glum2), st = gpu_sums(wf, s, "megakernel", spp)
+    p = wf.params(spp=spp, threads=8)
+    (crgb, clum, clum2), cst = wf.render_cpu_sums(s, 0, spp, p)
+    assert_bitwise(grgb, crgb, f"{name} rgb sums")
+    assert_bitwise(glum, clum, f"{name} lum sums")
+    assert_bitwise(glum2, clum2, f"{name} lum2 sums")
+    assert st.extension_rays == cst.extension_rays and st.shadow_rays == cst.shadow_rays
+    assert st.invalid_hits == 0
+
+
+@pytest.mark.parametrize("name,w,h", SMALL)
+def test_wavefront_equals_megakernel_and_oracle_trace(wf, po, scenes, name, w, h):
+    s = scenes(name, w, h)
+    spp = 2
+    mk, mst = gpu_sums(wf, s, "megakernel", spp)
+    r = wf.Renderer(s, "wavefront")
+    p = wf.params(spp=spp, trace_bounces=True)
+    r.reset()
+    st = r.run_passes(0, 1, p)
+    trace0 = r.trace()
+    st2 = r.run_passes(1, 1, p)
+    wfs = r.download_sums()
+    r.close()
+    for a, b, k in zip(wfs, mk, ("rgb", "lum", "lum2")):
+        assert_bitwise(a, b, f"{name} wavefront {k}")
+    assert st.extension_rays + st2.extension_rays == mst.extension_rays
+    assert st.shadow_rays + st2.shadow_rays == mst.shadow_rays
+    assert st.queue_overflows == 0 and st.invalid_hits == 0
+    # Integer work of sample 0 against the oracle's wavefront restatement.
+    o = po.OScene(s.arrays())
+    otrace, _, _ = o.wavefront_pass(0, 1234, 8)
+    assert trace0 == otrace, f"{name}: GPU {trace0[:3]} oracle {otrace[:3]}"
+    assert trace0[0][0] == w * h
+
+
+@pytest.mark.parametrize("name,w,h", SMALL)
+@pytest.mark.parametrize("pool", [0, 1024])
+def test_persistent_equals_megakernel(wf, scenes, name, w, h, pool):
+    s = scenes(name, w, h)
+    spp = 4
+    mk, mst = gpu_sums(wf, s, "megakernel", spp)
+    pe, st = gpu_sums(wf, s, "persistent", spp, pool_paths=pool)
+    for a, b, k in zip(pe, mk, ("rgb", "lum", "lum2")):
+        assert_bitwise(a, b, f"{name} persistent pool={pool} {k}")
+    assert st.extension_rays == mst.extension_rays and st.shadow_rays == mst.shadow_rays
+    assert st.queue_overflows == 0 and st.invalid_hits == 0
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (33, 17), (1, 720), (31, 1)])
+def test_odd_sizes_all_arms_agree(wf, scenes, w, h):
+    s = scenes("divergence", w, h)
+    ref, _ = gpu_sums(wf, s, "megakernel", 3)
+    for mode in ("wavefront", "persistent"):
+        got, _ = gpu_sums(wf, s, mode, 3)
+        for a, b in zip(got, ref):
+            assert_bitwise(a, b, f"{mode} {w}x{h}")
+    (crgb, _, _), _ = wf.render_cpu_sums(s, 0, 3, wf.params(spp=3, threads=4))
+    assert_bitwise(ref[0], crgb, f"cpu {w}x{h}")
+
+
+@pytest.mark.parametrize("mode", ["megakernel", "wavefront", "persistent"])
+def test_progressive_equals_one_shot(wf, scenes, mode):
+    s = scenes("sphere_grid", 40, 24)
+    r = wf.Renderer(s, mode)
+    p = wf.params(spp=6)
+    rgb, var, _ = r.render(p)
+    r.reset()
+    r.run_passes(0, 2, p)
+    r.run_passes(2, 4, p)
+    rgb2, var2 = r.download(6)
+    r.close()
+    assert_bitwise(rgb, rgb2, "rgb")
+    assert_bitwise(var, var2, "var")
+
+
+def test_max_depth_64_and_zero_light_scene(wf):
+    # Zero-light scene: one Lambert sphere, black environment, no emitters -> every pixel is 0.
+    sph = np.zeros((1, 8), np.float32)
+    sph[0, :4] = [0, 0, -3, 1]
+    mat = np.zeros((1, 12), np.float32)
+    mat[0, 4:7] = 0.5
+    env = np.zeros(20, np.float32)
+    cam = np.array([0, 0, 0, 0, 0, -1, 0, 1, 0, 60], np.float32)
+    s = wf.Scene.from_arrays(sph, np.zeros((0, 3)), np.zeros((0, 4)), mat, env, cam, 32, 18)
+    for mode in ("megakernel", "wavefront", "persistent"):
+        r = wf.Renderer(s, mode)
+        rgb, var, st = r.render(wf.params(spp=2, max_depth=64))
+        r.close()
+        assert not rgb.any() and not var.any(), mode
+        assert st.shadow_rays == 0
+    a, _ = gpu_sums(wf, wf.Scene("furnace", 24, 16), "megakernel", 2, depth=64)
+    b, _ = gpu_sums(wf, wf.Scene("furnace", 24, 16), "wavefront", 2, depth=64)
+    c, _ = gpu_sums(wf, wf.Scene("furnace", 24, 16), "persistent", 2, depth=64)
+    for x, y, z in zip(a, b, c):
+        assert_bitwise(x, y, "wavefront depth 64")
+        assert_bitwise(x, z, "persistent depth 64")
+
+
+def test_overflow_is_counted_and_renderer_survives(wf, scenes):
+    s = scenes("divergence", 64, 36)
+    r = wf.Renderer(s, "wavefront")
+    _, _, st = r.render(wf.params(spp=2, queue_capacity=100))
+    assert st.queue_overflows > 0
+    rgb, var, st2 = r.render(wf.params(spp=2))
+    r.close()
+    assert st2.queue_overflows == 0
+    ref, _ = gpu_sums(wf, s, "megakernel", 2)
+    assert_bitwise(rgb, wf.finalize(*ref, 2)[0], "after overflow")
+
+
+def test_unknown_mode_and_bad_params_raise(wf, scenes):
+    s = scenes("cornell", 8, 8)
+    with pytest.raises(wf.InvalidArgument):
+        wf.Renderer(s, 7)
+    r = wf.Renderer(s, "megakernel")
+    with pytest.raises(wf.InvalidArgument):
+        r.render(wf.params(spp=0))
+    rgb, _, _ = r.render(wf.params(spp=1))   # still usable
+    r.close()
+    assert np.isfinite(rgb).all()
+
+
+# ---- queue primitives (tests/test_gpu_queues.cu:61-406, tests/catch/gpu_primitives.cu) ----
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 1000, 2048, 4097, 921600, 1000003])
+@pytest.mark.parametrize("tile", [512, 1024, 2048, 4096])
+def test_scan_equals_host_prefix_sum(wf, n, tile):
+    import torch
+    rs = np.random.default_rng(n)
+    x = rs.integers(0, 5, size=n, dtype=np.uint32)
+    d_in = torch.from_numpy(x.astype(np.int32)).cuda()
+    d_out = torch.full((n + 1,), -559038737, dtype=torch.int32, device="cuda")   # 0xDEADBEEF canary
+    d_tot = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ws = C_ws(wf, n)
+    torch.cuda.synchronize()
+    wf._check(wf.lib().wf_scan_exclusive_u32(d_in.data_ptr(), d_out.data_ptr(), n, d_tot.data_ptr(), ws, None, tile))
+    torch.cuda.synchronize()
+    out = d_out.cpu().numpy().view(np.uint32)
+    want = np.concatenate([[0], np.cumsum(x, dtype=np.uint64)[:-1]]).astype(np.uint32)
+    assert np.array_equal(out[:n], want)
+    assert out[n] == 0xDEADBEEF
+    assert int(d_tot.cpu().numpy().view(np.uint32)[0]) == int(x.sum())
+    # in place
+    wf._check(wf.lib().wf_scan_exclusive_u32(d_in.data_ptr(), d_in.data_ptr(), n, None, ws, None, tile))
+    torch.cuda.synchronize()
+    assert np.array_equal(d_in.cpu().numpy().view(np.uint32), want)
+    wf.lib().wf_scan_workspace_destroy(ws)
+
+
+def C_ws(wf, cap):
+    import ctypes as C
+    h = C.c_void_p()
+    wf._check(wf.lib().wf_scan_workspace_create(cap, C.byref(h)))
+    return h
+
+
+@pytest.mark.parametrize("n", [1, 777, 65536, 921600, 1000003])
+def test_compact_equals_stable_filter(wf, n):
+    import torch
+    rs = np.random.default_rng(n + 1)
+    flags = (rs.random(n) < 0.37).astype(np.int32) * rs.integers(1, 9, size=n, dtype=np.int32)
+    vals = rs.integers(0, 2**31 - 1, size=n, dtype=np.int32)
+    d_f, d_v = torch.from_numpy(flags).cuda(), torch.from_numpy(vals).cuda()
+    d_o = torch.full((n + 1,), -559038737, dtype=torch.int32, device="cuda")
+    d_c = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ws = C_ws(wf, n)
+    wf._check(wf.lib().wf_compact_u32(d_v.data_ptr(), d_f.data_ptr(), d_o.data_ptr(), d_c.data_ptr(), None, n, ws,
+                                      None, 0))
+    torch.cuda.synchronize()
+    want = vals[flags != 0]
+    cnt = int(d_c.cpu()[0])
+    assert cnt == want.size
+    assert np.array_equal(d_o.cpu().numpy()[:cnt], want)
+    assert int(d_o.cpu().numpy().view(np.uint32)[cnt]) == 0xDEADBEEF
+    wf.lib().wf_scan_workspace_destroy(ws)
+
+
+def test_scan_rejects_n_above_capacity_without_side_effects(wf):
+    import torch
+    ws = C_ws(wf, 100)
+    d = torch.arange(200, dtype=torch.int32, device="cuda")
+    rc = wf.lib().wf_scan_exclusive_u32(d.data_ptr(), d.data_ptr(), 101, None, ws, None, 0)
+    torch.cuda.synchronize()
+    assert rc == wf.WF_ERR_INVALID
+    assert np.array_equal(d.cpu().numpy(), np.arange(200))
+    wf.lib().wf_scan_workspace_destroy(ws)
+
+
+@pytest.mark.parametrize("n,cap", [(1, 10), (1000, 2000), (100000, 200000), (100000, 5000)])
+def test_queue_push_count_and_contents(wf, n, cap):
+    import torch
+    rs = np.random.default_rng(cap)
+    flags = (rs.random(n) < 0.5).astype(np.int32)
+    d_f = torch.from_numpy(flags).cuda()
+    d_items = torch.full((cap + 1,), -559038737, dtype=torch.int32, device="cuda")
+    d_cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+    d_ovf = torch.zeros(1, dtype=torch.int32, device="cuda")
+    wf._check(wf.lib().wf_test_queue_push(d_f.data_ptr(), n, d_items.data_ptr(), cap, d_cnt.data_ptr(),
+                                          d_ovf.data_ptr(), None))
+    torch.cuda.synchronize()
+    want = np.flatnonzero(flags)
+    assert int(d_cnt.cpu()[0]) == want.size                    # counter counts every push
+    kept = min(want.size, cap)
+    assert int(d_ovf.cpu()[0]) == want.size - kept              # dropped pushes are counted
+    items = d_items.cpu().numpy()
+    assert int(items.view(np.uint32)[cap]) == 0xDEADBEEF
+    got = np.sort(items[:kept])
+    assert np.unique(got).size == kept and np.isin(got, want).all()
+    if kept == want.size:
+        assert np.array_equal(got, want)
+
+
+# ---- batch and serving: a job gives the same bits alone and in the mix ----
+def test_batch_jobs_equal_standalone_renders(wf, scenes):
+    specs = [(scenes("cornell555", 40, 30), 3, 1234, 8), (scenes("sphere_grid", 64, 36), 2, 99, 8),
+             (scenes("heightfield", 48, 27), 4, 5, 8), (scenes("divergence", 17, 9), 1, 1234, 8),
+             (scenes("sphere_grid", 64, 36), 3, 1234, 5)]
+    outs, st = wf.batch_render(specs, wf.params(pool_paths=4096))
+    assert st.queue_overflows == 0 and st.invalid_hits == 0
+    total_ext = 0
+    for (scene, spp, seed, depth), (rgb, var) in zip(specs, outs):
+        r = wf.Renderer(scene, "megakernel")
+        want, wvar, wst = r.render(wf.params(spp=spp, seed=seed, max_depth=depth))
+        r.close()
+        assert_bitwise(rgb, want, "batch rgb")
+        assert_bitwise(var, wvar, "batch var")
+        total_ext += wst.extension_rays
+    assert st.extension_rays == total_ext
+
+
+def test_serving_results_equal_alone_and_in_mix(wf, scenes):
+    srv = wf.Server(budget_bytes=8 << 30, tuning=wf.params(pool_paths=8192))
+    reqs = [(scenes("cornell555", 48, 27), 4, 11), (scenes("sphere_grid", 64, 36), 3, 12),
+            (scenes("heightfield", 32, 18), 2, 13), (scenes("sphere_grid", 64, 36), 5, 14),
+            (scenes("divergence", 40, 20), 2, 15)]
+    ids = []
+    for k, (s, spp, seed) in enumerate(reqs):
+        ids.append(srv.submit(s, spp, seed))
+        srv.step(1)   # later requests arrive while earlier ones are on the GPU
+    for _ in range(2000):
+        if all(srv.poll(i) == 1 for i in ids):
+            break
+        srv.step(1)
+    assert all(srv.poll(i) == 1 for i in ids)
+    st = srv.stats()
+    assert st["completed"] == len(reqs) and st["in_flight"] == 0
+    for rid, (s, spp, seed) in zip(ids, reqs):
+        rgb, var = srv.result(rid)
+        r = wf.Renderer(s, "megakernel")
+        want, wvar, _ = r.render(wf.params(spp=spp, seed=seed))
+        r.close()
+        assert_bitwise(rgb, want, "served rgb")
+        assert_bitwise(var, wvar, "served var")
+    srv.close()
+
+
+def test_serving_budget_rejects_and_defers(wf, scenes):
+    s = scenes("sphere_grid", 64, 36)
+    probe = wf.Server(budget_bytes=1 << 40, tuning=wf.params(pool_paths=4096))
+    pool = probe.stats()["bytes_in_use"]
+    probe.close()
+    req = 64 * 36 * 5 * 4 + 64 * 36 * 8 * 16          # film sums + sample buffer of one request
+    scene_bytes = 1 << 20
+    srv = wf.Server(budget_bytes=pool + scene_bytes + int(1.5 * req), tuning=wf.params(pool_paths=4096))
+    with pytest.raises(wf.WfError) as e:
+        srv.submit(wf.Scene("sphere_grid", 1920, 1080), 256)
+    assert e.value.code == wf.WF_ERR_BUDGET
+    ids = [srv.submit(s, 8, k) for k in range(4)]   # one fits at a time: admitted as memory frees
+    peak = 0
+    for _ in range(5000):
+        srv.step(1)
+        peak = max(peak, srv.stats()["in_flight"])
+        if all(srv.poll(i) == 1 for i in ids):
+            break
+    assert all(srv.poll(i) == 1 for i in ids)
+    assert peak == 1
+    srv.close()
+
+
+# ---- full-size properties on the material-divergent benchmark scene ----
+def test_full_size_sphere_grid_arms_agree(wf, scenes):
+    s = scenes("sphere_grid", 1280, 720)
+    spp = 8
+    mk, mst = gpu_sums(wf, s, "megakernel", spp)
+    pe, pst = gpu_sums(wf, s, "persistent", spp)
+    for a, b in zip(pe, mk):
+        assert_bitwise(a, b, "persistent full size")
+    assert pst.extension_rays == mst.extension_rays and pst.shadow_rays == mst.shadow_rays
+    rgb, var = wf.finalize(*mk, spp)
+    assert np.isfinite(rgb).all() and np.isfinite(var).all()
+    assert mst.invalid_hits == 0 and pst.queue_overflows == 0
