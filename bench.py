@@ -137,13 +137,16 @@ def main():
     print(f"[bench] megakernel median {percentile(mk_lat, 0.5):.2f} ms ({mk_mrays:.0f} Mrays/s); bitwise={bitwise}",
           file=sys.stderr, flush=True)
 
-    # Stage-timed render: per-kernel average launch durations and item counts.
+    # Stage-timed render (eager launches, HIP events on the engine's stream around each stage):
+    # per-kernel average launch durations and the items each launch processed.
     roof = None
-    try:
-        stages = r.stage_profile(0, spp, tuning) if hasattr(r, "stage_profile") else None
-    except Exception as e:   # noqa: BLE001
-        print(f"[bench] stage profile failed: {e}", file=sys.stderr)
-        stages = None
+    prof = wfamd.params(spp=spp, seed=seed, max_depth=depth, stage_timing=True, block_size=tuning.block_size,
+                        pool_paths=tuning.pool_paths, scan_tile=tuning.scan_tile)
+    r.reset()
+    pst = r.run_passes(0, spp, prof)
+    names = {"extend": "extend_classify", "shade_lambert": "shade", "shadow": "shadow", "compact": "retire"}
+    stages = {names[k]: v for k, v in pst.stages().items() if k in names}
+    print(f"[bench] stages: {json.dumps(stages)}", file=sys.stderr, flush=True)
     if stages:
         dom = max(stages, key=lambda k: stages[k]["ms"])
         d = stages[dom]

@@ -90,6 +90,10 @@ void to_stats(const wf::RenderStats& s, wf_render_stats* o) {
     o->stage_ms[6] = s.stages.compact;
     o->stage_ms[7] = s.stages.other;
     o->iterations = s.iterations;
+    for (int k = 0; k < 8; ++k) {
+        o->stage_items[k] = s.stages.items[k];
+        o->stage_launches[k] = s.stages.launches[k];
+    }
 }
 
 float bits_f(int v) {

@@ -19,6 +19,7 @@
 #include "wf_gpu.h"
 #include "wf_pathpool.h"
 #include "wf_queues.h"
+#include "wf_renderer.h"
 
 namespace wf::gpu {
 
@@ -88,8 +89,9 @@ public:
     // Renders every job to completion, adding samples [first_sample, first_sample + spp) to each
     // job's film. Jobs whose sample buffers would exceed the buffer budget are split into
     // consecutive sample chunks.
+    // With `stages`, the iterations run eagerly with HIP events around each stage (profiling).
     void render(const PersistentJob* jobs, int n_jobs, const PersistentTuning& t, DevCounters* ctr, hipStream_t stream,
-                PersistentResult* out);
+                PersistentResult* out, StageTimes* stages = nullptr);
 
     // Session API (continuous batching). open() sizes the pool; admit() appends a job (its
     // sample buffer is allocated here) and returns its index; run() replays the captured
@@ -109,7 +111,12 @@ public:
     int pool_paths() const { return cap_; }
 
     struct Impl;
+    void run_profiled(StageTimes* out);
+
 private:
+    struct LaunchPlan;
+    LaunchPlan make_plan(bool single);
+    void launch_iteration(const LaunchPlan& L, int par, hipEvent_t* ev);
     void ensure_pool(const PersistentTuning& t);
     void capture(bool single);
     uint64_t sbuf_budget_ = 4ull << 30;

@@ -602,64 +602,124 @@ int PersistentEngine::admit(const PersistentJob& job) {
     return (int)index;
 }
 
-void PersistentEngine::capture(bool single) {
+struct PersistentEngine::LaunchPlan {
+    PCtx c;
+    bool single;
+    size_t lds_bytes;
+    int gx, gs, gw, gr;
+};
+
+PersistentEngine::LaunchPlan PersistentEngine::make_plan(bool single) {
     Impl& m = *m_;
-    m.free_graph();
-    PCtx c = m.ctx;
-    c.n_lds = 0;
+    LaunchPlan L{};
+    L.single = single;
+    L.c = m.ctx;
+    L.c.n_lds = 0;
     if (single) {
-        c.scene0 = m.hscenes[0];
-        c.job0 = m.hjobs[0].dev;
-        c.n_lds = std::min(m.lds_nodes, c.scene0.n_nodes);
+        L.c.scene0 = m.hscenes[0];
+        L.c.job0 = m.hjobs[0].dev;
+        L.c.n_lds = std::min(m.lds_nodes, L.c.scene0.n_nodes);
     }
-    const size_t lds_bytes = (size_t)(single ? c.n_lds : m.lds_nodes) * sizeof(Node);
+    L.lds_bytes = (size_t)(single ? L.c.n_lds : m.lds_nodes) * sizeof(Node);
     const void* fx = single ? (const void*)pw_extend_classify_kernel<true> : (const void*)pw_extend_classify_kernel<false>;
     const void* fs = single ? (const void*)pw_shade_kernel<true> : (const void*)pw_shade_kernel<false>;
     const void* fw = single ? (const void*)pw_shadow_kernel<true> : (const void*)pw_shadow_kernel<false>;
-    const int gx = grid_for(fx, m.block, lds_bytes, m.cus);
-    const int gs = grid_for(fs, m.block, 0, m.cus);
-    const int gw = grid_for(fw, m.block, lds_bytes, m.cus);
-    const int gr = m.cus * 4;
-    check(hipStreamBeginCapture(m.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    for (int it = 0; it < m.graph_iters; ++it) {
-        const int par = it & 1;
-        pw_plan_kernel<<<1, 1, 0, m.stream>>>(m.st, m.ctr);
-        if (single) {
-            pw_extend_classify_kernel<true><<<gx, m.block, lds_bytes, m.stream>>>(c, par);
-            pw_shade_kernel<true><<<gs, m.block, 0, m.stream>>>(c);
-            pw_shadow_kernel<true><<<gw, m.block, lds_bytes, m.stream>>>(c);
-        } else {
-            pw_extend_classify_kernel<false><<<gx, m.block, lds_bytes, m.stream>>>(c, par);
-            pw_shade_kernel<false><<<gs, m.block, 0, m.stream>>>(c);
-            pw_shadow_kernel<false><<<gw, m.block, lds_bytes, m.stream>>>(c);
-        }
-        switch (m.items) {
-            case 4:
-                pw_retire_reduce_kernel<4><<<gr, kRetireThreads, 0, m.stream>>>(c, par);
-                pw_retire_scan_kernel<4><<<1, 1024, 0, m.stream>>>(c);
-                if (single) pw_retire_scatter_kernel<true, 4><<<gr, kRetireThreads, 0, m.stream>>>(c, par);
-                else pw_retire_scatter_kernel<false, 4><<<gr, kRetireThreads, 0, m.stream>>>(c, par);
-                break;
-            case 16:
-                pw_retire_reduce_kernel<16><<<gr, kRetireThreads, 0, m.stream>>>(c, par);
-                pw_retire_scan_kernel<16><<<1, 1024, 0, m.stream>>>(c);
-                if (single) pw_retire_scatter_kernel<true, 16><<<gr, kRetireThreads, 0, m.stream>>>(c, par);
-                else pw_retire_scatter_kernel<false, 16><<<gr, kRetireThreads, 0, m.stream>>>(c, par);
-                break;
-            default:
-                pw_retire_reduce_kernel<8><<<gr, kRetireThreads, 0, m.stream>>>(c, par);
-                pw_retire_scan_kernel<8><<<1, 1024, 0, m.stream>>>(c);
-                if (single) pw_retire_scatter_kernel<true, 8><<<gr, kRetireThreads, 0, m.stream>>>(c, par);
-                else pw_retire_scatter_kernel<false, 8><<<gr, kRetireThreads, 0, m.stream>>>(c, par);
-                break;
-        }
+    L.gx = grid_for(fx, m.block, L.lds_bytes, m.cus);
+    L.gs = grid_for(fs, m.block, 0, m.cus);
+    L.gw = grid_for(fw, m.block, L.lds_bytes, m.cus);
+    L.gr = m.cus * 4;
+    return L;
+}
+
+// One iteration; with `ev` (6 events) the stage boundaries are recorded for profiling:
+// ev[0] plan ev[1] extend_classify ev[2] shade ev[3] shadow ev[4] retire ev[5].
+void PersistentEngine::launch_iteration(const LaunchPlan& L, int par, hipEvent_t* ev) {
+    Impl& m = *m_;
+    hipStream_t st = m.stream;
+    const PCtx& c = L.c;
+    if (ev) check(hipEventRecord(ev[0], st), "hipEventRecord");
+    pw_plan_kernel<<<1, 1, 0, st>>>(m.st, m.ctr);
+    if (ev) check(hipEventRecord(ev[1], st), "hipEventRecord");
+    if (L.single) pw_extend_classify_kernel<true><<<L.gx, m.block, L.lds_bytes, st>>>(c, par);
+    else pw_extend_classify_kernel<false><<<L.gx, m.block, L.lds_bytes, st>>>(c, par);
+    if (ev) check(hipEventRecord(ev[2], st), "hipEventRecord");
+    if (L.single) pw_shade_kernel<true><<<L.gs, m.block, 0, st>>>(c);
+    else pw_shade_kernel<false><<<L.gs, m.block, 0, st>>>(c);
+    if (ev) check(hipEventRecord(ev[3], st), "hipEventRecord");
+    if (L.single) pw_shadow_kernel<true><<<L.gw, m.block, L.lds_bytes, st>>>(c);
+    else pw_shadow_kernel<false><<<L.gw, m.block, L.lds_bytes, st>>>(c);
+    if (ev) check(hipEventRecord(ev[4], st), "hipEventRecord");
+    switch (m.items) {
+        case 4:
+            pw_retire_reduce_kernel<4><<<L.gr, kRetireThreads, 0, st>>>(c, par);
+            pw_retire_scan_kernel<4><<<1, 1024, 0, st>>>(c);
+            if (L.single) pw_retire_scatter_kernel<true, 4><<<L.gr, kRetireThreads, 0, st>>>(c, par);
+            else pw_retire_scatter_kernel<false, 4><<<L.gr, kRetireThreads, 0, st>>>(c, par);
+            break;
+        case 16:
+            pw_retire_reduce_kernel<16><<<L.gr, kRetireThreads, 0, st>>>(c, par);
+            pw_retire_scan_kernel<16><<<1, 1024, 0, st>>>(c);
+            if (L.single) pw_retire_scatter_kernel<true, 16><<<L.gr, kRetireThreads, 0, st>>>(c, par);
+            else pw_retire_scatter_kernel<false, 16><<<L.gr, kRetireThreads, 0, st>>>(c, par);
+            break;
+        default:
+            pw_retire_reduce_kernel<8><<<L.gr, kRetireThreads, 0, st>>>(c, par);
+            pw_retire_scan_kernel<8><<<1, 1024, 0, st>>>(c);
+            if (L.single) pw_retire_scatter_kernel<true, 8><<<L.gr, kRetireThreads, 0, st>>>(c, par);
+            else pw_retire_scatter_kernel<false, 8><<<L.gr, kRetireThreads, 0, st>>>(c, par);
+            break;
     }
+    if (ev) check(hipEventRecord(ev[5], st), "hipEventRecord");
+}
+
+void PersistentEngine::capture(bool single) {
+    Impl& m = *m_;
+    m.free_graph();
+    const LaunchPlan L = make_plan(single);
+    check(hipStreamBeginCapture(m.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    for (int it = 0; it < m.graph_iters; ++it) launch_iteration(L, it & 1, nullptr);
     hipError_t e = hipStreamEndCapture(m.stream, &m.graph);
     check(e, "hipStreamEndCapture");
     check(hipGetLastError(), "persistent capture");
     check(hipGraphInstantiate(&m.exec, m.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
     m.graph_single = single;
     m.graph_scenes = (int)m.hscenes.size();
+}
+
+// Eager iterations with HIP events around each stage (synchronising once per iteration), until
+// the admitted work is done. Items per stage come from the control block of each iteration.
+void PersistentEngine::run_profiled(StageTimes* out) {
+    Impl& m = *m_;
+    if (m.hjobs.empty()) return;
+    const bool single = m.hjobs.size() == 1 && m.hscenes.size() == 1;
+    const LaunchPlan L = make_plan(single);
+    hipEvent_t ev[6];
+    for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+    for (int it = 0;; ++it) {
+        launch_iteration(L, it & 1, ev);
+        check(hipMemcpyAsync(m.st_host, m.st, sizeof(PState), hipMemcpyDeviceToHost, m.stream), "copy state");
+        check(hipStreamSynchronize(m.stream), "profile sync");
+        const PState& s = *m.st_host;
+        float ms[5];
+        for (int k = 0; k < 5; ++k) check(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]), "hipEventElapsedTime");
+        if (s.done) break;
+        out->other += ms[0];
+        out->extend += ms[1];
+        out->shade_lambert += ms[2];
+        out->shadow += ms[3];
+        out->compact += ms[4];
+        out->items[0] += s.it_act;
+        out->items[2] += (long long)s.q[0] + s.q[1] + s.q[2];
+        out->items[5] += s.q[3];
+        out->items[6] += s.it_act;
+        out->items[7] += 1;
+        out->launches[0] += 1;
+        out->launches[2] += 1;
+        out->launches[5] += 1;
+        out->launches[6] += 3;
+        out->launches[7] += 1;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
 }
 
 void PersistentEngine::run(int launches) {
@@ -706,7 +766,7 @@ void PersistentEngine::finish_job(int job) {
 }
 
 void PersistentEngine::render(const PersistentJob* jobs, int n_jobs, const PersistentTuning& t, DevCounters* ctr,
-                              hipStream_t stream, PersistentResult* out) {
+                              hipStream_t stream, PersistentResult* out, StageTimes* stages) {
     PersistentResult res;
     // Split jobs into consecutive sample chunks so that one chunk's sample buffers fit the budget.
     std::vector<PersistentJob> rest(jobs, jobs + n_jobs);
@@ -740,11 +800,16 @@ void PersistentEngine::render(const PersistentJob* jobs, int n_jobs, const Persi
         if (res.chunks > 0) reset_jobs();
         for (const PersistentJob& c : chunk) admit(c);
         PersistentProgress pr;
-        for (;;) {
-            run(1);
-            res.graph_launches += 1;
+        if (stages) {
+            run_profiled(stages);
             pr = progress();
-            if (pr.done) break;
+        } else {
+            for (;;) {
+                run(1);
+                res.graph_launches += 1;
+                pr = progress();
+                if (pr.done) break;
+            }
         }
         res.iterations += (int)pr.iterations;
         for (int k = 0; k < (int)m_->hjobs.size(); ++k) finish_job(k);

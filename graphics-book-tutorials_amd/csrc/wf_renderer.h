@@ -30,6 +30,9 @@ struct RenderParams {
 struct StageTimes {   // milliseconds summed over a render
     double extend = 0, classify = 0, shade_lambert = 0, shade_metal = 0, shade_dielectric = 0, shadow = 0, compact = 0,
            other = 0;
+    // Items processed and kernel launches per stage, same order as the fields above.
+    long long items[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    long long launches[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 struct RenderStats {

@@ -147,7 +147,7 @@ void GpuRenderer::run_passes(int first_sample, int count, const RenderParams& p,
             tune.scan_tile = p.scan_tile;
             tune.lds_nodes = p.lds_nodes;
             gpu::PersistentResult pr;
-            m.pe->render(&job, 1, tune, m.d_ctr, m.stream, &pr);
+            m.pe->render(&job, 1, tune, m.d_ctr, m.stream, &pr, p.stage_timing ? &st : nullptr);
             iterations = pr.iterations;
         }
     }
@@ -175,6 +175,10 @@ void GpuRenderer::run_passes(int first_sample, int count, const RenderParams& p,
         stats->stages.shadow += st.shadow;
         stats->stages.compact += st.compact;
         stats->stages.other += st.other;
+        for (int k = 0; k < 8; ++k) {
+            stats->stages.items[k] += st.items[k];
+            stats->stages.launches[k] += st.launches[k];
+        }
     }
 }
 
@@ -256,6 +260,7 @@ DeviceFacts query_device() {
     DeviceFacts f;
     f.name = prop.name;
     f.arch = prop.gcnArchName;
+    if (f.name.empty()) f.name = "AMD Instinct (" + f.arch.substr(0, f.arch.find(':')) + ")";
     f.driver_version = std::to_string(drv);
     f.runtime_version = std::to_string(rt);
     f.compute_units = prop.multiProcessorCount;

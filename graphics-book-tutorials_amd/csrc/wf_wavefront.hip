@@ -303,6 +303,17 @@ void WavefrontEngine::run_pass(const DeviceFilm& film, DevCounters* ctr, uint32_
         check(hipMemcpyAsync(&next, counters_ + kCntActive, 4, hipMemcpyDeviceToHost, stream), "copy active count");
         check(hipStreamSynchronize(stream), "wavefront bounce sync");
         mark(&StageTimes::compact);
+        if (timing) {
+            uint32_t q[4];
+            check(hipMemcpy(q, counters_, sizeof(q), hipMemcpyDeviceToHost), "copy queue counts");
+            const long long items[8] = {n_active, n_active, std::min(q[0], cap), std::min(q[1], cap),
+                                        std::min(q[2], cap), std::min(q[3], cap), n_active, 0};
+            const long long launches[8] = {1, 1, 1, 1, 1, 1, 5, 2};
+            for (int k = 0; k < 8; ++k) {
+                stages->items[k] += items[k];
+                stages->launches[k] += launches[k];
+            }
+        }
         if (trace != nullptr) {
             BounceTrace bt{};
             uint32_t r[8] = {0, 0, 0, 0, 0, 0, 0, 0};

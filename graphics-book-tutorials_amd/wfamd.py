@@ -40,13 +40,19 @@ class RenderStats(C.Structure):
     _fields_ = [("render_ms", C.c_double), ("total_ms", C.c_double), ("extension_rays", C.c_uint64),
                 ("shadow_rays", C.c_uint64), ("queue_high_water", C.c_int64 * 5), ("queue_overflows", C.c_int64),
                 ("invalid_hits", C.c_int64), ("peak_vram_bytes", C.c_uint64), ("stage_ms", C.c_double * 8),
-                ("iterations", C.c_int64)]
+                ("iterations", C.c_int64), ("stage_items", C.c_int64 * 8), ("stage_launches", C.c_int64 * 8)]
+
+    STAGES = ("extend", "classify", "shade_lambert", "shade_metal", "shade_dielectric", "shadow", "compact", "other")
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
-        d["queue_high_water"] = list(self.queue_high_water)
-        d["stage_ms"] = list(self.stage_ms)
+        for k in ("queue_high_water", "stage_ms", "stage_items", "stage_launches"):
+            d[k] = list(getattr(self, k))
         return d
+
+    def stages(self):
+        return {n: {"ms": self.stage_ms[i], "items": self.stage_items[i], "launches": self.stage_launches[i]}
+                for i, n in enumerate(self.STAGES) if self.stage_launches[i] > 0}
 
 
 class SceneInfo(C.Structure):

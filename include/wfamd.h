@@ -54,7 +54,10 @@ typedef struct wf_render_params {
 } wf_render_params;
 
 /* RenderStats, include/wf/renderer.hpp:23-37. stage_ms: extend, classify, shade_lambert,
- * shade_metal, shade_dielectric, shadow, compact, other. */
+ * shade_metal, shade_dielectric, shadow, compact, other. With stage_timing set, stage_items and
+ * stage_launches give the items each stage processed (paths, queue entries or shadow rays) and
+ * its kernel launches. The persistent engine fuses classify into extend and shades the three
+ * materials in one kernel, reported under extend and shade_lambert. */
 typedef struct wf_render_stats {
     double render_ms;
     double total_ms;
@@ -66,6 +69,8 @@ typedef struct wf_render_stats {
     uint64_t peak_vram_bytes;
     double stage_ms[8];
     int64_t iterations;
+    int64_t stage_items[8];
+    int64_t stage_launches[8];
 } wf_render_stats;
 
 typedef struct wf_scene_info {

@@ -214,29 +214,7 @@ def test_unknown_mode_and_bad_params_raise(wf, scenes):
 
 
 # ---- queue primitives (tests/test_gpu_queues.cu:61-406, tests/catch/gpu_primitives.cu) ----
-@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 1000, 2048, 4097, 921600, 1000003])
-@pytest.mark.parametrize("tile", [512, 1024, 2048, 4096])
-def test_scan_equals_host_prefix_sum(wf, n, tile):
-    import torch
-    rs = np.random.default_rng(n)
-    x = rs.integers(0, 5, size=n, dtype=np.uint32)
-    d_in = torch.from_numpy(x.astype(np.int32)).cuda()
-    d_out = torch.full((n + 1,), -559038737, dtype=torch.int32, device="cuda")   # 0xDEADBEEF canary
-    d_tot = torch.zeros(1, dtype=torch.int32, device="cuda")
-    ws = C_ws(wf, n)
-    torch.cuda.synchronize()
-    wf._check(wf.lib().wf_scan_exclusive_u32(d_in.data_ptr(), d_out.data_ptr(), n, d_tot.data_ptr(), ws, None, tile))
-    torch.cuda.synchronize()
-    out = d_out.cpu().numpy().view(np.uint32)
-    want = np.concatenate([[0], np.cumsum(x, dtype=np.uint64)[:-1]]).astype(np.uint32)
-    assert np.array_equal(out[:n], want)
-    assert out[n] == 0xDEADBEEF
-    assert int(d_tot.cpu().numpy().view(np.uint32)[0]) == int(x.sum())
-    # in place
-    wf._check(wf.lib().wf_scan_exclusive_u32(d_in.data_ptr(), d_in.data_ptr(), n, None, ws, None, tile))
-    torch.cuda.synchronize()
-    assert np.array_equal(d_in.cpu().numpy().view(np.uint32), want)
-    wf.lib().wf_scan_workspace_destroy(ws)
+CANARY = 0xDEADBEEF
 
 
 def C_ws(wf, cap):
@@ -246,56 +224,75 @@ def C_ws(wf, cap):
     return h
 
 
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 1000, 2048, 4097, 921600, 1000003])
+@pytest.mark.parametrize("tile", [512, 1024, 2048, 4096])
+def test_scan_equals_host_prefix_sum(wf, n, tile):
+    from hipmem import DevArray, sync
+    rs = np.random.default_rng(n)
+    x = rs.integers(0, 5, size=n, dtype=np.uint32)
+    d_in = DevArray(x)
+    d_out = DevArray(np.full(n + 1, CANARY, np.uint32))
+    d_tot = DevArray(np.zeros(1, np.uint32))
+    ws = C_ws(wf, n)
+    wf._check(wf.lib().wf_scan_exclusive_u32(d_in.ptr, d_out.ptr, n, d_tot.ptr, ws, None, tile))
+    sync()
+    out = d_out.get()
+    want = np.concatenate([[0], np.cumsum(x, dtype=np.uint64)[:-1]]).astype(np.uint32)
+    assert np.array_equal(out[:n], want)
+    assert out[n] == CANARY
+    assert int(d_tot.get()[0]) == int(x.sum())
+    wf._check(wf.lib().wf_scan_exclusive_u32(d_in.ptr, d_in.ptr, n, None, ws, None, tile))   # in place
+    assert np.array_equal(d_in.get(), want)
+    wf.lib().wf_scan_workspace_destroy(ws)
+
+
 @pytest.mark.parametrize("n", [1, 777, 65536, 921600, 1000003])
 def test_compact_equals_stable_filter(wf, n):
-    import torch
+    from hipmem import DevArray
     rs = np.random.default_rng(n + 1)
-    flags = (rs.random(n) < 0.37).astype(np.int32) * rs.integers(1, 9, size=n, dtype=np.int32)
-    vals = rs.integers(0, 2**31 - 1, size=n, dtype=np.int32)
-    d_f, d_v = torch.from_numpy(flags).cuda(), torch.from_numpy(vals).cuda()
-    d_o = torch.full((n + 1,), -559038737, dtype=torch.int32, device="cuda")
-    d_c = torch.zeros(1, dtype=torch.int32, device="cuda")
+    flags = ((rs.random(n) < 0.37) * rs.integers(1, 9, size=n)).astype(np.uint32)
+    vals = rs.integers(0, 2**31 - 1, size=n, dtype=np.uint32)
+    d_f, d_v = DevArray(flags), DevArray(vals)
+    d_o = DevArray(np.full(n + 1, CANARY, np.uint32))
+    d_c = DevArray(np.zeros(1, np.uint32))
     ws = C_ws(wf, n)
-    wf._check(wf.lib().wf_compact_u32(d_v.data_ptr(), d_f.data_ptr(), d_o.data_ptr(), d_c.data_ptr(), None, n, ws,
-                                      None, 0))
-    torch.cuda.synchronize()
+    wf._check(wf.lib().wf_compact_u32(d_v.ptr, d_f.ptr, d_o.ptr, d_c.ptr, None, n, ws, None, 0))
     want = vals[flags != 0]
-    cnt = int(d_c.cpu()[0])
+    cnt = int(d_c.get()[0])
+    out = d_o.get()
     assert cnt == want.size
-    assert np.array_equal(d_o.cpu().numpy()[:cnt], want)
-    assert int(d_o.cpu().numpy().view(np.uint32)[cnt]) == 0xDEADBEEF
+    assert np.array_equal(out[:cnt], want)
+    assert out[cnt] == CANARY
     wf.lib().wf_scan_workspace_destroy(ws)
 
 
 def test_scan_rejects_n_above_capacity_without_side_effects(wf):
-    import torch
+    from hipmem import DevArray
     ws = C_ws(wf, 100)
-    d = torch.arange(200, dtype=torch.int32, device="cuda")
-    rc = wf.lib().wf_scan_exclusive_u32(d.data_ptr(), d.data_ptr(), 101, None, ws, None, 0)
-    torch.cuda.synchronize()
+    x = np.arange(200, dtype=np.uint32)
+    d = DevArray(x)
+    rc = wf.lib().wf_scan_exclusive_u32(d.ptr, d.ptr, 101, None, ws, None, 0)
     assert rc == wf.WF_ERR_INVALID
-    assert np.array_equal(d.cpu().numpy(), np.arange(200))
+    assert np.array_equal(d.get(), x)
     wf.lib().wf_scan_workspace_destroy(ws)
 
 
 @pytest.mark.parametrize("n,cap", [(1, 10), (1000, 2000), (100000, 200000), (100000, 5000)])
 def test_queue_push_count_and_contents(wf, n, cap):
-    import torch
+    from hipmem import DevArray
     rs = np.random.default_rng(cap)
-    flags = (rs.random(n) < 0.5).astype(np.int32)
-    d_f = torch.from_numpy(flags).cuda()
-    d_items = torch.full((cap + 1,), -559038737, dtype=torch.int32, device="cuda")
-    d_cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
-    d_ovf = torch.zeros(1, dtype=torch.int32, device="cuda")
-    wf._check(wf.lib().wf_test_queue_push(d_f.data_ptr(), n, d_items.data_ptr(), cap, d_cnt.data_ptr(),
-                                          d_ovf.data_ptr(), None))
-    torch.cuda.synchronize()
-    want = np.flatnonzero(flags)
-    assert int(d_cnt.cpu()[0]) == want.size                    # counter counts every push
+    flags = (rs.random(n) < 0.5).astype(np.uint32)
+    d_f = DevArray(flags)
+    d_items = DevArray(np.full(cap + 1, CANARY, np.uint32))
+    d_cnt = DevArray(np.zeros(1, np.uint32))
+    d_ovf = DevArray(np.zeros(1, np.uint32))
+    wf._check(wf.lib().wf_test_queue_push(d_f.ptr, n, d_items.ptr, cap, d_cnt.ptr, d_ovf.ptr, None))
+    want = np.flatnonzero(flags).astype(np.uint32)
+    assert int(d_cnt.get()[0]) == want.size                    # the counter counts every push
     kept = min(want.size, cap)
-    assert int(d_ovf.cpu()[0]) == want.size - kept              # dropped pushes are counted
-    items = d_items.cpu().numpy()
-    assert int(items.view(np.uint32)[cap]) == 0xDEADBEEF
+    assert int(d_ovf.get()[0]) == want.size - kept              # dropped pushes are counted
+    items = d_items.get()
+    assert items[cap] == CANARY
     got = np.sort(items[:kept])
     assert np.unique(got).size == kept and np.isin(got, want).all()
     if kept == want.size:
@@ -352,7 +349,7 @@ def test_serving_budget_rejects_and_defers(wf, scenes):
     pool = probe.stats()["bytes_in_use"]
     probe.close()
     req = 64 * 36 * 5 * 4 + 64 * 36 * 8 * 16          # film sums + sample buffer of one request
-    scene_bytes = 1 << 20
+    scene_bytes = 200_000                              # sphere_grid upload is ~115 KB
     srv = wf.Server(budget_bytes=pool + scene_bytes + int(1.5 * req), tuning=wf.params(pool_paths=4096))
     with pytest.raises(wf.WfError) as e:
         srv.submit(wf.Scene("sphere_grid", 1920, 1080), 256)
