@@ -27,13 +27,12 @@ METRIC = "Mrays/s at 1280x720x64spp depth 8 per scene; p50/p99 render latency"
 HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes per item of each persistent-engine kernel (device layout of
 # graphics-book-tutorials_amd/csrc/wf_pathpool.h and wf_persistent.hip):
-#   extend_classify per live path: active index 4, ray + throughput + radiance 64, radiance 16,
-#                   surface 32, alive flag 4, queue entry 4 (the BVH stays in LDS / L2)
-#   shade per queue entry: entry 4, surface 32, ray + throughput 48, key 8, write ray + throughput 48, flag 4
+#   extend per live path: active index 4, ray 32, hit 8, queue entry 4 (BVH and primitives in LDS/L2)
+#   shade per queue entry: entry 4, work id 8, ray + throughput + radiance + key 72, hit 8, then
+#         ray + throughput 48 and radiance (or the sample) 16, next-active or free entry 4,
+#         and 48 per shadow ray pushed (counted here at the Lambert share, ~0.6 x 48 ~ 28)
 #   shadow per shadow ray: entry 48, radiance read-modify-write 32
-#   retire per live path: index 4 + flag 4 (reduce), index 4 + flag 4 + work id 8, then 4 (survivor)
-#                   or 4 + radiance 16 + sample 16 (finished)
-BYTES_PER_ITEM = {"extend_classify": 124, "shade": 144, "shadow": 80, "retire": 36}
+BYTES_PER_ITEM = {"extend": 48, "shade": 188, "shadow": 80}
 
 
 def percentile(v, p):
@@ -97,8 +96,8 @@ def main():
     if not args.no_autotune:
         t0 = time.perf_counter()
         tuned = wfamd.autotune([(scene, spp, seed, depth)], probe_spp=4)
-        tuning.block_size, tuning.pool_paths, tuning.scan_tile = tuned.block_size, tuned.pool_paths, tuned.scan_tile
-        print(f"[bench] autotune: block {tuned.block_size} pool {tuned.pool_paths} tile {tuned.scan_tile} "
+        tuning.block_size, tuning.pool_paths, tuning.lds_kib = tuned.block_size, tuned.pool_paths, tuned.lds_kib
+        print(f"[bench] autotune: block {tuned.block_size} pool {tuned.pool_paths} lds {tuned.lds_kib} KiB "
               f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
 
     r = wfamd.Renderer(scene, "persistent")
@@ -141,10 +140,10 @@ def main():
     # per-kernel average launch durations and the items each launch processed.
     roof = None
     prof = wfamd.params(spp=spp, seed=seed, max_depth=depth, stage_timing=True, block_size=tuning.block_size,
-                        pool_paths=tuning.pool_paths, scan_tile=tuning.scan_tile)
+                        pool_paths=tuning.pool_paths, lds_kib=tuning.lds_kib)
     r.reset()
     pst = r.run_passes(0, spp, prof)
-    names = {"extend": "extend_classify", "shade_lambert": "shade", "shadow": "shadow", "compact": "retire"}
+    names = {"extend": "extend", "shade_lambert": "shade", "shadow": "shadow"}
     stages = {names[k]: v for k, v in pst.stages().items() if k in names}
     print(f"[bench] stages: {json.dumps(stages)}", file=sys.stderr, flush=True)
     if stages:
@@ -169,7 +168,7 @@ def main():
         "data": "synthetic: procedural sphere_grid scene (1024 jittered spheres, seeded), no external data",
         "config": {"workload": f"{args.scene} {args.width}x{args.height} {spp}spp depth {depth} seed {seed} "
                                f"(material-divergent scene)", "arm": "persistent batched wavefront",
-                   "block_size": tuning.block_size, "pool_paths": tuning.pool_paths, "scan_tile": tuning.scan_tile,
+                   "block_size": tuning.block_size, "pool_paths": tuning.pool_paths, "lds_kib": tuning.lds_kib,
                    "parallelism": "single GPU"},
         "p50_ms": percentile(lat, 0.5), "p99_ms": percentile(lat, 0.99),
         "extension_mrays_per_s": ext / elapsed / 1e6, "rays_per_render": rays_per_render,

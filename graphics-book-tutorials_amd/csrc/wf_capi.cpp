@@ -62,6 +62,7 @@ wf::RenderParams to_params(const wf_render_params* p) {
     r.pool_paths = p->pool_paths;
     r.scan_tile = p->scan_tile;
     r.lds_nodes = p->lds_nodes;
+    r.lds_kib = p->lds_kib;
     return r;
 }
 
@@ -603,8 +604,7 @@ wf::gpu::PersistentTuning to_tuning(const wf_render_params* p) {
     if (p) {
         t.block_size = p->block_size;
         t.pool_paths = p->pool_paths;
-        t.scan_tile = p->scan_tile;
-        t.lds_nodes = p->lds_nodes;
+        t.lds_kib = p->lds_kib;
     }
     return t;
 }
@@ -678,7 +678,7 @@ extern "C" int wf_batch_render(const wf_job* jobs, int n_jobs, const wf_render_p
             stats->graph_launches = pr.graph_launches;
             stats->block_size = t.block_size;
             stats->pool_paths = eng.pool_paths();
-            stats->scan_tile = t.scan_tile;
+            stats->lds_kib = t.lds_kib;
             stats->device_bytes = eng.bytes() + b.scenes.bytes();
         }
         return (int)WF_OK;
@@ -693,7 +693,7 @@ extern "C" int wf_autotune(const wf_job* jobs, int n_jobs, int probe_spp, wf_ren
         wf::gpu::PersistentTuning best = wf::gpu::autotune_persistent(b.jobs.data(), n_jobs, b.stream, probe_spp, nullptr);
         tuning->block_size = best.block_size;
         tuning->pool_paths = best.pool_paths;
-        tuning->scan_tile = best.scan_tile;
+        tuning->lds_kib = best.lds_kib;
         return (int)WF_OK;
     });
 }
@@ -810,10 +810,10 @@ int wf_server_step(wf_server* s, int launches) {
         }
         if (!s->running.empty() && launches > 0) {
             s->eng.run(launches);
-            wf::gpu::PersistentProgress pr = s->eng.progress();
+            s->eng.progress();
             std::vector<wf_server::Req*> still;
             for (wf_server::Req* r : s->running) {
-                if (s->eng.job_complete(r->engine_job, pr)) {
+                if (s->eng.job_complete(r->engine_job)) {
                     s->eng.finish_job(r->engine_job);
                     r->film.download(r->job.spp, r->job.rgb, r->job.lum_var, s->stream);
                     r->film.release();

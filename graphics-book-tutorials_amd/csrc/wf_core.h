@@ -327,11 +327,22 @@ WF_HD bool hit_prim(const SceneView& s, const Prim& p, const Ray& r, float* t) {
     return hit_triangle(p.p0, p.p1, p.p2, r, t);
 }
 
-// Node sources: plain memory, or (on the device) an LDS copy of the first nodes.
+// Scene sources for traversal: where nodes come from and how a leaf primitive is tested. MemNodes
+// reads both from the scene arrays; the device adds sources that serve the top of the tree and
+// compact primitive records from LDS. Every source tests with the same hit_sphere/hit_triangle.
 struct MemNodes {
     const Node* nodes;
-    WF_HD Node operator()(int n) const { return nodes[n]; }
+    const Prim* prims;
+    int n_spheres;
+    WF_HD Node node(int n) const { return nodes[n]; }
+    WF_HD bool hit(int slot, const Ray& r, float* t) const {
+        const Prim& p = prims[slot];
+        if (p.id < n_spheres) return hit_sphere(p.p0, p.radius, r, t);
+        return hit_triangle(p.p0, p.p1, p.p2, r, t);
+    }
 };
+
+WF_HD MemNodes mem_nodes(const SceneView& s) { return MemNodes{s.nodes, s.prims, s.n_spheres}; }
 
 // Stackless skip-link traversal (include/wf/bvh.hpp:91-116): on a box hit descend to the left
 // child (leaf: test its primitives, then follow skip); on a miss follow skip. tmax shrinks with
@@ -344,14 +355,14 @@ WF_HD bool intersect_with(const SceneView& s, const NodeSrc& src, Ray r, Hit* h,
     Vec3 inv_d{1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};
     int n = 0;
     while (n >= 0 && n < s.n_nodes) {
-        const Node node = src(n);
+        const Node node = src.node(n);
         if (hit_box(node.lo, node.hi, r.o, inv_d, r.tmin, r.tmax)) {
             if (node.child < 0) {
                 const int code = -(node.child + 1);
                 const int first = code >> 2, cnt = (code & 3) + 1;
                 for (int i = 0; i < cnt; ++i) {
                     float t;
-                    if (hit_prim(s, s.prims[first + i], r, &t)) {
+                    if (src.hit(first + i, r, &t)) {
                         r.tmax = t;
                         h->t = t;
                         h->prim = first + i;
@@ -370,7 +381,7 @@ WF_HD bool intersect_with(const SceneView& s, const NodeSrc& src, Ray r, Hit* h,
 }
 
 WF_HD bool intersect(const SceneView& s, const Ray& r, Hit* h, bool anyhit = false) {
-    return intersect_with(s, MemNodes{s.nodes}, r, h, anyhit);
+    return intersect_with(s, mem_nodes(s), r, h, anyhit);
 }
 
 // Oracle over every primitive in slot order. Slot order differs from id order, so ties between
@@ -765,7 +776,7 @@ WF_HD Vec3 trace_path_with(const SceneView& s, const NodeSrc& src, int px, int p
 
 WF_HD Vec3 trace_path(const SceneView& s, int px, int py, uint32_t sample, uint32_t seed,
                       int max_depth, RayCount* rc) {
-    return trace_path_with(s, MemNodes{s.nodes}, px, py, sample, seed, max_depth, rc);
+    return trace_path_with(s, mem_nodes(s), px, py, sample, seed, max_depth, rc);
 }
 
 }  // namespace wf

@@ -104,13 +104,42 @@ __device__ __forceinline__ void wave_add_u32(unsigned int* dst, uint32_t v) {
     if (lane_id() == 0 && s) atomicAdd(dst, s);
 }
 
-// Node source that serves the first n_lds nodes (the breadth-first top of the tree) from LDS.
+// Scene source that serves the first n_lds nodes (the breadth-first top of the tree) from LDS and,
+// when lds_prims is set, every leaf primitive from a compact LDS record: centre and radius of a
+// sphere, or the first vertex of a triangle with w = -1 (the other two vertices then come from
+// memory). Spheres always have a positive radius (validated), so w < 0 marks a triangle.
 struct LdsNodes {
     const Node* lds;
     const Node* mem;
     int n_lds;
-    __device__ __forceinline__ Node operator()(int n) const { return n < n_lds ? lds[n] : mem[n]; }
+    const Prim* prims;
+    int n_spheres;
+    const float4* lds_prims = nullptr;
+    __device__ __forceinline__ Node node(int n) const { return n < n_lds ? lds[n] : mem[n]; }
+    __device__ __forceinline__ bool hit(int slot, const Ray& r, float* t) const {
+        if (lds_prims) {
+            const float4 q = lds_prims[slot];
+            if (q.w >= 0.0f) return hit_sphere(Vec3{q.x, q.y, q.z}, q.w, r, t);
+            const Prim& p = prims[slot];
+            return hit_triangle(Vec3{q.x, q.y, q.z}, p.p1, p.p2, r, t);
+        }
+        const Prim& p = prims[slot];
+        if (p.id < n_spheres) return hit_sphere(p.p0, p.radius, r, t);
+        return hit_triangle(p.p0, p.p1, p.p2, r, t);
+    }
 };
+
+__device__ __forceinline__ LdsNodes lds_nodes_of(const SceneView& s, const Node* lds, int n_lds,
+                                                 const float4* lds_prims = nullptr) {
+    return LdsNodes{lds, s.nodes, n_lds, s.prims, s.n_spheres, lds_prims};
+}
+
+// Compact LDS record of leaf slot `slot` (see LdsNodes).
+__device__ __forceinline__ float4 lds_prim_record(const SceneView& s, int slot) {
+    const Prim& p = s.prims[slot];
+    return p.id < s.n_spheres ? make_float4(p.p0.x, p.p0.y, p.p0.z, p.radius)
+                              : make_float4(p.p0.x, p.p0.y, p.p0.z, -1.0f);
+}
 
 // Stages the first n nodes into LDS; the whole block must call it.
 __device__ __forceinline__ void stage_nodes(Node* lds, const Node* nodes, int n) {

@@ -23,7 +23,7 @@ __global__ __launch_bounds__(256) void megakernel_kernel(SceneView s, DeviceFilm
         const int p = py * s.width + px;
         float rgb[3] = {film.rgb[3 * p], film.rgb[3 * p + 1], film.rgb[3 * p + 2]};
         float lum = film.lum[p], lum2 = film.lum2[p];
-        const LdsNodes src{lds_nodes, s.nodes, n_lds};
+        const LdsNodes src = lds_nodes_of(s, lds_nodes, n_lds);
         for (int k = 0; k < count; ++k) {
             RayCount rc{0, 0, 0};
             Vec3 L = trace_path_with(s, src, px, py, first_sample + (uint32_t)k, seed, max_depth, &rc);

@@ -1,15 +1,23 @@
 // Persistent batched wavefront. A fixed pool of path slots is kept full: each iteration
 // regenerates freed slots from a global work counter over all admitted jobs (work item = one
-// (job, sample, pixel) path, sample-major inside a job), traces and classifies every live path,
-// shades the three material queues, traces the shadow queue and retires finished paths with a
-// stable two-way partition (survivors -> next active list, finished -> free-slot stack). Every
-// stage kernel runs on a persistent grid and reads its item count from a device control block,
-// so an iteration needs no host round trip; K iterations are captured once into a hipGraph and
-// replayed until the control block reports that all work is done.
+// (job, sample, pixel) path, sample-major inside a job, pixels in 8x8 tiles), then
+//   extend  traverses every live path (BVH top and, when they fit, compact primitive records in
+//           LDS), stores the hit and pushes the path into one of four queues by the material it
+//           hit: terminal (miss, emitter, invalid hit), Lambert, metal, dielectric;
+//   shade   classifies and shades the four queues as one index space (a wave sees one queue
+//           except at the boundaries); survivors are pushed to the next active list, finished
+//           paths write their radiance and push their slot to the free stack, Lambert pushes
+//           shadow rays;
+//   shadow  traces the shadow queue (any hit), adds unoccluded contributions and finishes paths
+//           that ended with a shadow ray pending.
+// All pushes are wave64-aggregated (ballot + mbcnt, one atomic per wave). Every stage kernel runs
+// on a persistent grid and reads its item count from a device control block, so an iteration
+// needs no host round trip; K iterations are captured once into a hipGraph and replayed until the
+// control block reports that all admitted work is done.
 //
-// A finished path writes its radiance to its job's sample buffer at (sample, pixel); when a job
-// has no work left, a resolve kernel adds the samples to the film in sample order, so the film
-// holds exactly the bits that per-sample passes (megakernel, plain wavefront) produce.
+// A finished path writes its radiance to its job's sample buffer at (sample, pixel); when a job's
+// paths have all finished, a resolve kernel adds the samples to the film in sample order, so the
+// film holds exactly the bits that per-sample passes (megakernel, plain wavefront) produce.
 #pragma once
 #include <cstdint>
 #include <memory>
@@ -24,6 +32,7 @@
 namespace wf::gpu {
 
 constexpr int kMaxScenes = 8;
+constexpr int kMaxJobs = 4096;
 
 struct PersistentJob {
     const SceneView* scene = nullptr;   // view with device pointers (one per distinct scene)
@@ -35,10 +44,9 @@ struct PersistentJob {
 };
 
 struct PersistentTuning {
-    int block_size = 0;    // threads per workgroup (0 -> 256)
+    int block_size = 0;    // threads per workgroup: 128, 256 or 512 (0 -> 512)
     int pool_paths = 0;    // path slots = queue capacity (0 -> 2^22)
-    int scan_tile = 0;     // retire partition tile (0 -> 2048)
-    int lds_nodes = -1;    // BVH nodes staged in LDS per workgroup (-1 -> 1024)
+    int lds_kib = 0;       // LDS per workgroup for BVH nodes and primitive records (0 -> 64 KiB)
     int graph_iters = 0;   // iterations per captured graph (0 -> 16; rounded up to even)
 };
 
@@ -58,24 +66,28 @@ struct JobDev {
     DeviceFilm film;
 };
 
-// Device control block.
+// Device control block. Queues: 0 terminal, 1 Lambert, 2 metal, 3 dielectric, 4 shadow.
 struct PState {
-    unsigned long long next_work, total_work, it_work_base, min_live_work;
+    unsigned long long next_work, total_work, it_work_base, pad64;
     uint32_t n_live, n_free, n_jobs, have_prev;
     uint32_t it_live_old, it_new, it_act, it_free_after;
-    uint32_t q[4];   // lambert, metal, dielectric, shadow
-    uint32_t n_alive, iterations, done, overflow;
+    uint32_t q[5];
+    uint32_t n_next, n_freed, iterations, done, overflow;
 };
 
+// LDS layout of a workgroup: nodes of each scene, then primitive records of each scene.
 struct LdsPlan {
     int n;
-    int off[kMaxScenes];
-    int cnt[kMaxScenes];
+    int node_off[kMaxScenes];
+    int node_cnt[kMaxScenes];
+    int prim_off[kMaxScenes];
+    int prim_cnt[kMaxScenes];   // 0 or the scene's primitive count
+    int total_nodes, total_prims;
 };
 
 // Host mirror of the control block fields the serving loop polls.
 struct PersistentProgress {
-    unsigned long long next_work = 0, total_work = 0, min_live_work = 0;
+    unsigned long long next_work = 0, total_work = 0;
     uint32_t n_live = 0, done = 0, iterations = 0;
 };
 
@@ -88,30 +100,29 @@ public:
 
     // Renders every job to completion, adding samples [first_sample, first_sample + spp) to each
     // job's film. Jobs whose sample buffers would exceed the buffer budget are split into
-    // consecutive sample chunks.
-    // With `stages`, the iterations run eagerly with HIP events around each stage (profiling).
+    // consecutive sample chunks. With `stages`, the iterations run eagerly with HIP events around
+    // each stage (profiling) instead of graph replays.
     void render(const PersistentJob* jobs, int n_jobs, const PersistentTuning& t, DevCounters* ctr, hipStream_t stream,
                 PersistentResult* out, StageTimes* stages = nullptr);
 
     // Session API (continuous batching). open() sizes the pool; admit() appends a job (its
     // sample buffer is allocated here) and returns its index; run() replays the captured
-    // iteration graph `launches` times; progress() reads the control block; finish_job()
-    // resolves a job whose work is complete (work_end <= min(next_work, min_live_work)) into
-    // its film and frees its sample buffer. reset_jobs() clears the job table once all are done.
+    // iteration graph `launches` times; progress() reads the control block and the per-job
+    // finished-path counts; finish_job() resolves a complete job into its film and frees its
+    // sample buffer; reset_jobs() clears the job table once nothing is in flight.
     void open(const PersistentTuning& t, DevCounters* ctr, hipStream_t stream);
     int admit(const PersistentJob& job);
     void run(int launches);
+    void run_profiled(StageTimes* out);
     PersistentProgress progress();
-    bool job_complete(int job, const PersistentProgress& pr) const;
+    bool job_complete(int job) const;
     void finish_job(int job);
     void reset_jobs();
-    uint64_t job_buffer_bytes(const PersistentJob& job) const;
     uint64_t pool_bytes() const { return pool_bytes_; }
     uint64_t bytes() const;
     int pool_paths() const { return cap_; }
 
     struct Impl;
-    void run_profiled(StageTimes* out);
 
 private:
     struct LaunchPlan;
@@ -119,13 +130,14 @@ private:
     void launch_iteration(const LaunchPlan& L, int par, hipEvent_t* ev);
     void ensure_pool(const PersistentTuning& t);
     void capture(bool single);
+    void update_lds_plan();
     uint64_t sbuf_budget_ = 4ull << 30;
     int cap_ = 0;
     uint64_t pool_bytes_ = 0;
     std::unique_ptr<Impl> m_;
 };
 
-// Picks block size, pool size and scan tile for a workload by timing short renders of it.
+// Picks block size, pool size and LDS budget for a workload by timing short renders of it.
 PersistentTuning autotune_persistent(const PersistentJob* jobs, int n_jobs, hipStream_t stream, int probe_spp,
                                      std::vector<std::pair<PersistentTuning, double>>* trials);
 

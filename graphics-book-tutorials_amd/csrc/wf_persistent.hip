@@ -1,16 +1,13 @@
 // Persistent batched wavefront engine; the design is described in wf_persistent.h.
 // Per iteration (all counts live in the device control block PState):
-//   plan            1 thread: fold the previous iteration's results, size this one
-//   extend_classify regenerate freed slots, traverse every live path, classify, push queues
-//   shade           the three material queues as one index space (a wave sees one material
-//                   except at the two boundaries); Lambert pushes shadow rays
-//   shadow          any-hit traversal of the shadow queue, add unoccluded contributions
-//   retire          reduce / scan / scatter: survivors -> next active list (stable),
-//                   finished -> free-slot stack and their radiance -> the job's sample buffer
-// The stage semantics and the order of L accumulation are those of the plain wavefront
-// (src/gpu/wavefront.cu:88-270), so every path computes the same bits as in the megakernel.
+//   plan    1 thread: fold the previous iteration's results, size this one
+//   extend  regenerate freed slots, traverse every live path, push it by hit material
+//   shade   classify + shade the four queues; re-queue survivors, finish the others
+//   shadow  any-hit shadow rays; finish paths whose last bounce left a shadow ray pending
+// Stage semantics and the order of L accumulation are those of the plain wavefront
+// (src/gpu/wavefront.cu:88-270): emission in classify, then the unoccluded NEE contribution
+// before the next classify, so every path computes the megakernel's bits.
 #include <algorithm>
-#include <chrono>
 #include <cstring>
 
 #include "wf_persistent.h"
@@ -19,32 +16,41 @@ namespace wf::gpu {
 
 namespace {
 
-constexpr int kMaxJobs = 4096;
-constexpr int kRetireThreads = 256;
 constexpr int kDefaultPool = 1 << 22;
-constexpr int kDefaultBlock = 256;
+constexpr int kDefaultBlock = 512;
 constexpr int kDefaultGraphIters = 16;
-constexpr int kDefaultLdsNodes = 1024;
+constexpr int kDefaultLdsKiB = 64;
 
 struct PCtx {
     PState* st;
     JobDev* jobs;
+    uint32_t* job_done;   // finished paths per job
     SceneView* scenes;
     LdsPlan* plan;
     PathPool P;
     ShadowQueue SQ;
-    uint2* wk;          // (job, local work index) per slot
-    uint32_t* alive;
-    uint32_t* A[2];     // active lists (ping-pong by iteration parity)
-    uint32_t* F;        // free-slot stack
-    uint32_t* Q[3];     // material queues
-    uint32_t* partial;  // retire tile partials
+    uint2* wk;            // (job, local work index) per slot
+    uint32_t* A[2];       // active lists (ping-pong by iteration parity)
+    uint32_t* F;          // free-slot stack
+    uint32_t* Q[4];       // terminal, Lambert, metal, dielectric queues
     DevCounters* ctr;
     uint32_t cap;
-    int n_lds;          // single mode: nodes staged in LDS
-    SceneView scene0;   // single mode: the scene
-    JobDev job0;        // single mode: the job
+    LdsPlan plan0;        // single mode: the LDS layout
+    SceneView scene0;     // single mode: the scene
+    JobDev job0;          // single mode: the job
 };
+
+template <bool SINGLE>
+__device__ __forceinline__ const SceneView& scene_of(const PCtx& c, int sc) {
+    if constexpr (SINGLE) return c.scene0;
+    else return c.scenes[sc];
+}
+
+template <bool SINGLE>
+__device__ __forceinline__ const JobDev& job_of(const PCtx& c, uint32_t jid) {
+    if constexpr (SINGLE) return c.job0;
+    else return c.jobs[jid];
+}
 
 __device__ __forceinline__ int find_job(const JobDev* jobs, uint32_t n_jobs, unsigned long long w) {
     int lo = 0, hi = (int)n_jobs - 1;
@@ -56,25 +62,54 @@ __device__ __forceinline__ int find_job(const JobDev* jobs, uint32_t n_jobs, uns
     return lo;
 }
 
-template <bool SINGLE>
-__device__ __forceinline__ void stage_lds(const PCtx& c, Node* lds, LdsPlan* sp) {
-    if (SINGLE) {
-        stage_nodes(lds, c.scene0.nodes, c.n_lds);
-    } else {
-        if (threadIdx.x == 0) *sp = *c.plan;
-        __syncthreads();
-        for (int sc = 0; sc < sp->n; ++sc) {
-            const Node* src = c.scenes[sc].nodes;
-            for (int i = threadIdx.x; i < sp->cnt[sc]; i += blockDim.x) lds[sp->off[sc] + i] = src[i];
-        }
-        __syncthreads();
-    }
+// Pixel of position q in a pass, visiting the image in 8-row bands of 8x8 tiles (edge tiles
+// smaller), so the 64 consecutive new paths of a wave start from one 8x8 pixel tile.
+__device__ __forceinline__ void tile_pixel(uint32_t q, uint32_t W, uint32_t H, uint32_t* px, uint32_t* py) {
+    const uint32_t b = q / (8u * W);
+    const uint32_t r = q - b * 8u * W;
+    const uint32_t hb = min(8u, H - 8u * b);
+    const uint32_t j = r / (8u * hb);
+    const uint32_t in = r - j * 8u * hb;
+    const uint32_t wb = min(8u, W - 8u * j);
+    *px = 8u * j + in % wb;
+    *py = 8u * b + in / wb;
 }
 
+// Stages the LDS layout (BVH tops and primitive records); the whole block calls it.
 template <bool SINGLE>
-__device__ __forceinline__ LdsNodes node_src(const PCtx& c, const Node* lds, const LdsPlan* sp, int sc) {
-    if (SINGLE) return LdsNodes{lds, c.scene0.nodes, c.n_lds};
-    return LdsNodes{lds + sp->off[sc], c.scenes[sc].nodes, sp->cnt[sc]};
+__device__ __forceinline__ void stage_lds(const PCtx& c, Node* lds, LdsPlan* sp) {
+    if constexpr (SINGLE) {
+        if (threadIdx.x == 0) *sp = c.plan0;
+    } else {
+        if (threadIdx.x == 0) *sp = *c.plan;
+    }
+    __syncthreads();
+    float4* lprim = reinterpret_cast<float4*>(lds + sp->total_nodes);
+    for (int sc = 0; sc < sp->n; ++sc) {
+        const SceneView& s = scene_of<SINGLE>(c, sc);
+        for (int i = threadIdx.x; i < sp->node_cnt[sc]; i += blockDim.x) lds[sp->node_off[sc] + i] = s.nodes[i];
+        for (int i = threadIdx.x; i < sp->prim_cnt[sc]; i += blockDim.x)
+            lprim[sp->prim_off[sc] + i] = lds_prim_record(s, i);
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ LdsNodes src_of(const SceneView& s, const Node* lds, const LdsPlan* sp, int sc) {
+    const float4* lprim = reinterpret_cast<const float4*>(lds + sp->total_nodes);
+    return lds_nodes_of(s, lds + sp->node_off[sc], sp->node_cnt[sc],
+                        sp->prim_cnt[sc] > 0 ? lprim + sp->prim_off[sc] : nullptr);
+}
+
+// Adds one per lane with pred to job_done[jid], one atomic per distinct job id in the wave.
+__device__ __forceinline__ void count_done(uint32_t* job_done, bool pred, uint32_t jid) {
+    uint64_t m = __ballot(pred);
+    while (m) {
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        const uint32_t j0 = __shfl(jid, leader);
+        const uint64_t same = __ballot(pred && jid == j0);
+        if ((int)lane_id() == leader) atomicAdd(&job_done[j0], (uint32_t)__popcll(same));
+        m &= ~same;
+    }
 }
 
 __global__ void pw_init_kernel(PState* st, uint32_t* F, uint32_t cap) {
@@ -83,13 +118,13 @@ __global__ void pw_init_kernel(PState* st, uint32_t* F, uint32_t cap) {
     if (i == 0) {
         PState z{};
         z.n_free = cap;
-        z.min_live_work = ~0ull;
         *st = z;
     }
 }
 
-__global__ void pw_admit_kernel(PState* st, JobDev* jobs, JobDev job, uint32_t index) {
+__global__ void pw_admit_kernel(PState* st, JobDev* jobs, uint32_t* job_done, JobDev job, uint32_t index) {
     jobs[index] = job;
+    job_done[index] = 0u;
     st->n_jobs = index + 1;
     st->total_work = job.work_end;
     st->done = 0;
@@ -97,15 +132,15 @@ __global__ void pw_admit_kernel(PState* st, JobDev* jobs, JobDev job, uint32_t i
 
 __global__ void pw_plan_kernel(PState* st, DevCounters* ctr) {
     if (st->have_prev) {
-        const uint32_t act = st->it_act, a = st->n_alive;
+        const uint32_t act = st->it_act;
         ctr->extension_rays += act;
-        ctr->shadow_rays += st->q[3];
+        ctr->shadow_rays += st->q[4];
         for (int k = 0; k < 4; ++k)
-            if (st->q[k] > ctr->high_water[k]) ctr->high_water[k] = st->q[k];
+            if (st->q[k + 1] > ctr->high_water[k]) ctr->high_water[k] = st->q[k + 1];
         if (act > ctr->high_water[4]) ctr->high_water[4] = act;
         ctr->overflows += st->overflow;
-        st->n_live = a;
-        st->n_free = st->it_free_after + (act - a);
+        st->n_live = st->n_next;
+        st->n_free = st->it_free_after + st->n_freed;
     }
     st->have_prev = 1;
     const unsigned long long rem = st->total_work - st->next_work;
@@ -116,16 +151,16 @@ __global__ void pw_plan_kernel(PState* st, DevCounters* ctr) {
     st->it_free_after = st->n_free - n_new;
     st->it_work_base = st->next_work;
     st->next_work += n_new;
-    st->q[0] = st->q[1] = st->q[2] = st->q[3] = 0;
-    st->n_alive = 0;
+    for (int k = 0; k < 5; ++k) st->q[k] = 0;
+    st->n_next = 0;
+    st->n_freed = 0;
     st->overflow = 0;
-    st->min_live_work = ~0ull;
     if (st->it_act > 0) st->iterations += 1;
     st->done = st->it_act == 0 ? 1u : 0u;
 }
 
 template <bool SINGLE>
-__global__ __launch_bounds__(512) void pw_extend_classify_kernel(PCtx c, int parity) {
+__global__ __launch_bounds__(512) void pw_extend_kernel(PCtx c, int parity) {
     extern __shared__ Node lds[];
     __shared__ LdsPlan sp;
     stage_lds<SINGLE>(c, lds, &sp);
@@ -140,93 +175,109 @@ __global__ __launch_bounds__(512) void pw_extend_classify_kernel(PCtx c, int par
         const uint32_t idx = b0 + threadIdx.x;
         const bool valid = idx < n_act;
         uint32_t slot = 0;
-        int mat = -1;
+        int cat = -1;
         if (valid) {
-            int jid = 0;
-            PathState ps;
+            uint32_t jid = 0;
+            Ray r;
             if (idx >= n_old) {
                 const uint32_t k = idx - n_old;
                 slot = c.F[free_after + k];
                 A[idx] = slot;
                 const unsigned long long w = wbase + k;
-                if (!SINGLE) jid = find_job(c.jobs, n_jobs, w);
-                const JobDev& job = SINGLE ? c.job0 : c.jobs[jid];
-                const SceneView& s = SINGLE ? c.scene0 : c.scenes[job.scene];
+                if constexpr (!SINGLE) jid = (uint32_t)find_job(c.jobs, n_jobs, w);
+                const JobDev& job = job_of<SINGLE>(c, jid);
+                const SceneView& s = scene_of<SINGLE>(c, job.scene);
                 const uint32_t local = (uint32_t)(w - job.work_begin);
                 const uint32_t WH = (uint32_t)(job.width * job.height);
                 const uint32_t sample = job.first_sample + local / WH;
-                const uint32_t pix = local % WH;
-                ps = init_path(s, (int)(pix % (uint32_t)job.width), (int)(pix / (uint32_t)job.width), sample, job.seed);
-                store_scatter(c.P, slot, ps);
-                c.P.key[slot] = make_uint2(ps.key.k0, ps.key.k1);
-                c.wk[slot] = make_uint2((uint32_t)jid, local);
+                uint32_t px, py;
+                tile_pixel(local % WH, (uint32_t)job.width, (uint32_t)job.height, &px, &py);
+                const PathState ps = init_path(s, (int)px, (int)py, sample, job.seed);
+                store_new_path(c.P, slot, ps);   // ray, flags, T = 1, L = 0, key
+                // Sample-buffer index (sample, pixel) of this path.
+                c.wk[slot] = make_uint2(jid, (local / WH) * WH + py * (uint32_t)job.width + px);
+                r = ps.ray;
             } else {
                 slot = A[idx];
-                ps = load_path(c.P, slot, true);
-                if (!SINGLE) jid = (int)c.wk[slot].x;
+                const float4 o = c.P.ro[slot], d = c.P.rd[slot];
+                r.o = Vec3{o.x, o.y, o.z};
+                r.d = Vec3{d.x, d.y, d.z};
+                r.tmin = kRayEps;
+                r.tmax = kInf;
+                if constexpr (!SINGLE) jid = c.wk[slot].x;
             }
-            const JobDev& job = SINGLE ? c.job0 : c.jobs[jid];
-            const SceneView& s = SINGLE ? c.scene0 : c.scenes[job.scene];
+            const int sc = job_of<SINGLE>(c, jid).scene;
+            const SceneView& s = scene_of<SINGLE>(c, sc);
             Hit h;
-            intersect_with(s, node_src<SINGLE>(c, lds, &sp, job.scene), ps.ray, &h, false);
+            intersect_with(s, src_of(s, lds, &sp, sc), r, &h, false);
             if (!hit_is_valid(s, h)) {
                 inv += 1;
                 h.prim = -1;
             }
-            Surface f;
-            mat = classify(s, ps, h, job.max_depth, &f);
-            c.P.L[slot] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
-            if (mat >= 0) store_surface(c.P, slot, f);
-            c.alive[slot] = mat >= 0 ? 1u : 0u;
+            c.P.hit[slot] = make_float2(h.t, __int_as_float(h.prim));
+            if (h.prim < 0) {
+                cat = 0;
+            } else {
+                const int mt = s.materials[s.prims[h.prim].material].type;
+                cat = mt == kEmissive ? 0 : mt + 1;
+            }
         }
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const bool mine = valid && mat == q;
-            const uint32_t e = wave_push(c.Q[q], &st->q[q], c.cap, slot, mine, &st->overflow);
-            if (mine && e == 0xFFFFFFFFu) c.alive[slot] = 0u;
-        }
+        for (int q = 0; q < 4; ++q) wave_push(c.Q[q], &st->q[q], c.cap, slot, valid && cat == q, &st->overflow);
     }
     wave_add_u32(&c.ctr->invalid_hits, inv);
 }
 
 template <bool SINGLE>
-__global__ __launch_bounds__(512) void pw_shade_kernel(PCtx c) {
+__global__ __launch_bounds__(512) void pw_shade_kernel(PCtx c, int parity) {
     PState* st = c.st;
-    const uint32_t nL = st->q[0], nM = st->q[1], nD = st->q[2];
-    const uint32_t total = nL + nM + nD;
+    const uint32_t n0 = st->q[0], n1 = st->q[1], n2 = st->q[2], n3 = st->q[3];
+    const uint32_t e1 = n0 + n1, e2 = e1 + n2, total = e2 + n3;
+    uint32_t* An = parity ? c.A[0] : c.A[1];
+    uint32_t* Ffree = c.F + st->it_free_after;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < total; b0 += stride) {
         const uint32_t idx = b0 + threadIdx.x;
-        int has = 0;
+        const bool valid = idx < total;
+        uint32_t slot = 0, jid = 0;
+        bool live = false, fin = false;
+        int has = 0, sc = 0;
         Ray sh;
         Vec3 contrib;
-        uint32_t slot = 0;
-        int sc = 0;
-        if (idx < total) {
-            int q;
+        if (valid) {
+            const uint32_t* Qp;
             uint32_t qi;
-            if (idx < nL) { q = 0; qi = idx; }
-            else if (idx < nL + nM) { q = 1; qi = idx - nL; }
-            else { q = 2; qi = idx - nL - nM; }
-            const uint32_t* Qp = q == 0 ? c.Q[0] : (q == 1 ? c.Q[1] : c.Q[2]);
+            if (idx < n0) { Qp = c.Q[0]; qi = idx; }
+            else if (idx < e1) { Qp = c.Q[1]; qi = idx - n0; }
+            else if (idx < e2) { Qp = c.Q[2]; qi = idx - e1; }
+            else { Qp = c.Q[3]; qi = idx - e2; }
             slot = Qp[qi];
-            const int jid = SINGLE ? 0 : (int)c.wk[slot].x;
-            const JobDev& job = SINGLE ? c.job0 : c.jobs[jid];
-            const SceneView& s = SINGLE ? c.scene0 : c.scenes[job.scene];
+            const uint2 w = c.wk[slot];
+            if constexpr (!SINGLE) jid = w.x;
+            const JobDev& job = job_of<SINGLE>(c, jid);
             sc = job.scene;
-            const Surface f = load_surface(c.P, slot);
-            PathState ps = load_path(c.P, slot, false);
-            bool live;
-            if (q == 0) live = shade_lambert(s, ps, f, &sh, &contrib, &has);
-            else if (q == 1) live = shade_metal(s, ps, f);
-            else live = shade_dielectric(s, ps, f);
-            store_scatter(c.P, slot, ps);
-            c.alive[slot] = live ? 1u : 0u;
+            const SceneView& s = scene_of<SINGLE>(c, sc);
+            PathState ps = load_path(c.P, slot, true);
+            const float2 hv = c.P.hit[slot];
+            const Hit h{hv.x, __float_as_int(hv.y)};
+            Surface f;
+            const int mat = classify(s, ps, h, job.max_depth, &f);
+            if (mat == kLambert) live = shade_lambert(s, ps, f, &sh, &contrib, &has);
+            else if (mat == kMetal) live = shade_metal(s, ps, f);
+            else if (mat == kDielectric) live = shade_dielectric(s, ps, f);
+            if (live) store_scatter(c.P, slot, ps);
+            const float4 L = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
+            fin = !live && !has;
+            if (fin) job.sbuf[w.y] = L;
+            else c.P.L[slot] = L;
         }
-        const uint32_t e = wave_reserve(&st->q[3], has != 0);
+        wave_push(An, &st->n_next, c.cap, slot, live, &st->overflow);
+        wave_push(Ffree, &st->n_freed, c.cap, slot, fin, &st->overflow);
+        count_done(c.job_done, fin, jid);
+        const uint32_t e = wave_reserve(&st->q[4], has != 0);
         if (has) {
             c.SQ.o[e] = make_float4(sh.o.x, sh.o.y, sh.o.z, sh.tmax);
-            c.SQ.d[e] = make_float4(sh.d.x, sh.d.y, sh.d.z, __int_as_float((int)slot));
+            c.SQ.d[e] = make_float4(sh.d.x, sh.d.y, sh.d.z, __int_as_float((int)(slot | (live ? 0u : 0x80000000u))));
             c.SQ.c[e] = make_float4(contrib.x, contrib.y, contrib.z, __int_as_float(sc));
         }
     }
@@ -237,149 +288,46 @@ __global__ __launch_bounds__(512) void pw_shadow_kernel(PCtx c) {
     extern __shared__ Node lds[];
     __shared__ LdsPlan sp;
     stage_lds<SINGLE>(c, lds, &sp);
-    const uint32_t n = c.st->q[3];
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += stride) {
-        const float4 o = c.SQ.o[idx], d = c.SQ.d[idx], cc = c.SQ.c[idx];
-        const int sc = SINGLE ? 0 : __float_as_int(cc.w);
-        const SceneView& s = SINGLE ? c.scene0 : c.scenes[sc];
-        Ray r;
-        r.o = Vec3{o.x, o.y, o.z};
-        r.d = Vec3{d.x, d.y, d.z};
-        r.tmin = kRayEps;
-        r.tmax = o.w;
-        Hit h;
-        if (!intersect_with(s, node_src<SINGLE>(c, lds, &sp, sc), r, &h, true)) {
-            const uint32_t slot = (uint32_t)__float_as_int(d.w);
-            float4 l = c.P.L[slot];
-            l.x += cc.x;
-            l.y += cc.y;
-            l.z += cc.z;
-            c.P.L[slot] = l;
-        }
-    }
-}
-
-// Block-wide exclusive scan of per-thread counts (256 threads).
-__device__ __forceinline__ uint32_t retire_block_scan(uint32_t t, uint32_t* total) {
-    __shared__ uint32_t wt[kRetireThreads / 64];
-    const int lane = (int)lane_id(), w = threadIdx.x >> 6;
-    uint32_t x = t;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) wt[w] = x;
-    __syncthreads();
-    uint32_t off = 0, tot = 0;
-#pragma unroll
-    for (int k = 0; k < kRetireThreads / 64; ++k) {
-        const uint32_t v = wt[k];
-        if (k < w) off += v;
-        tot += v;
-    }
-    __syncthreads();
-    *total = tot;
-    return off + x - t;
-}
-
-template <int ITEMS>
-__global__ __launch_bounds__(kRetireThreads) void pw_retire_reduce_kernel(PCtx c, int parity) {
-    const uint32_t n = c.st->it_act;
-    const uint32_t tile = kRetireThreads * ITEMS;
-    const uint32_t n_tiles = (n + tile - 1) / tile;
-    const uint32_t* A = parity ? c.A[1] : c.A[0];
-    for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-        const uint32_t i0 = t * tile + threadIdx.x * ITEMS;
-        uint32_t cnt = 0;
-#pragma unroll
-        for (int k = 0; k < ITEMS; ++k)
-            if (i0 + k < n) cnt += c.alive[A[i0 + k]];
-        uint32_t total;
-        retire_block_scan(cnt, &total);
-        if (threadIdx.x == 0) c.partial[t] = total;
-    }
-}
-
-template <int ITEMS>
-__global__ __launch_bounds__(1024) void pw_retire_scan_kernel(PCtx c) {
-    __shared__ uint32_t wtot[16];
-    __shared__ uint32_t carry;
-    const uint32_t n = c.st->it_act;
-    const uint32_t tile = kRetireThreads * ITEMS;
-    const uint32_t nb = (n + tile - 1) / tile;
-    if (threadIdx.x == 0) carry = 0u;
-    __syncthreads();
-    const int w = threadIdx.x >> 6, lane = (int)lane_id();
-    for (uint32_t base = 0; base < nb; base += 1024) {
-        const uint32_t i = base + threadIdx.x;
-        const uint32_t v = i < nb ? c.partial[i] : 0u;
-        uint32_t x = v;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(x, o);
-            if (lane >= o) x += y;
-        }
-        if (lane == 63) wtot[w] = x;
-        __syncthreads();
-        uint32_t off = 0, tot = 0;
-        for (int k = 0; k < 16; ++k) {
-            const uint32_t y = wtot[k];
-            if (k < w) off += y;
-            tot += y;
-        }
-        const uint32_t cv = carry;
-        if (i < nb) c.partial[i] = cv + off + x - v;
-        __syncthreads();
-        if (threadIdx.x == 0) carry = cv + tot;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) c.st->n_alive = carry;
-}
-
-template <bool SINGLE, int ITEMS>
-__global__ __launch_bounds__(kRetireThreads) void pw_retire_scatter_kernel(PCtx c, int parity) {
     PState* st = c.st;
-    const uint32_t n = st->it_act, free_after = st->it_free_after;
-    const uint32_t tile = kRetireThreads * ITEMS;
-    const uint32_t n_tiles = (n + tile - 1) / tile;
-    const uint32_t* A = parity ? c.A[1] : c.A[0];
-    uint32_t* An = parity ? c.A[0] : c.A[1];
-    for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-        const uint32_t i0 = t * tile + threadIdx.x * ITEMS;
-        uint32_t slot[ITEMS], fl[ITEMS];
-        uint32_t cnt = 0;
-#pragma unroll
-        for (int k = 0; k < ITEMS; ++k) {
-            slot[k] = i0 + k < n ? A[i0 + k] : 0u;
-            fl[k] = i0 + k < n ? c.alive[slot[k]] : 0u;
-            cnt += fl[k];
-        }
-        uint32_t total;
-        uint32_t pos = retire_block_scan(cnt, &total) + c.partial[t];
-        unsigned long long min_work = ~0ull;
-#pragma unroll
-        for (int k = 0; k < ITEMS; ++k) {
-            const uint32_t idx = i0 + k;
-            if (idx >= n) continue;
-            const uint2 wk = c.wk[slot[k]];
-            const JobDev& job = SINGLE ? c.job0 : c.jobs[wk.x];
-            if (fl[k]) {
-                An[pos++] = slot[k];
-                const unsigned long long gw = job.work_begin + wk.y;
-                min_work = gw < min_work ? gw : min_work;
-            } else {
-                c.F[free_after + (idx - pos)] = slot[k];
-                job.sbuf[wk.y] = c.P.L[slot[k]];
+    const uint32_t n = st->q[4];
+    uint32_t* Ffree = c.F + st->it_free_after;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n; b0 += stride) {
+        const uint32_t idx = b0 + threadIdx.x;
+        uint32_t slot = 0, jid = 0;
+        bool fin = false;
+        if (idx < n) {
+            const float4 o = c.SQ.o[idx], d = c.SQ.d[idx], cc = c.SQ.c[idx];
+            const uint32_t tag = (uint32_t)__float_as_int(d.w);
+            slot = tag & 0x7fffffffu;
+            fin = (tag >> 31) != 0;
+            const int sc = SINGLE ? 0 : __float_as_int(cc.w);
+            const SceneView& s = scene_of<SINGLE>(c, sc);
+            Ray r;
+            r.o = Vec3{o.x, o.y, o.z};
+            r.d = Vec3{d.x, d.y, d.z};
+            r.tmin = kRayEps;
+            r.tmax = o.w;
+            Hit h;
+            const bool blocked = intersect_with(s, src_of(s, lds, &sp, sc), r, &h, true);
+            if (!blocked || fin) {
+                float4 l = c.P.L[slot];
+                if (!blocked) {
+                    l.x += cc.x;
+                    l.y += cc.y;
+                    l.z += cc.z;
+                }
+                if (fin) {
+                    const uint2 w = c.wk[slot];
+                    if constexpr (!SINGLE) jid = w.x;
+                    job_of<SINGLE>(c, jid).sbuf[w.y] = l;
+                } else {
+                    c.P.L[slot] = l;
+                }
             }
         }
-        // Wave minimum, then one atomic per wave.
-        for (int o = 32; o > 0; o >>= 1) {
-            const unsigned long long y = __shfl_xor(min_work, o);
-            min_work = y < min_work ? y : min_work;
-        }
-        if (lane_id() == 0 && min_work != ~0ull) atomicMin(&st->min_live_work, min_work);
+        wave_push(Ffree, &st->n_freed, c.cap, slot, fin, &st->overflow);
+        count_done(c.job_done, fin, jid);
     }
 }
 
@@ -412,17 +360,17 @@ int grid_for(const void* fn, int block, size_t lds, int cus) {
 struct PersistentEngine::Impl {
     hipStream_t stream = nullptr;
     DevCounters* ctr = nullptr;
-    PersistentTuning tune;
-    int block = kDefaultBlock, items = 8, graph_iters = kDefaultGraphIters, lds_nodes = kDefaultLdsNodes, cus = 0;
+    int block = kDefaultBlock, graph_iters = kDefaultGraphIters, lds_bytes = kDefaultLdsKiB * 1024, cus = 0;
     std::vector<void*> pool_allocs;
     PState* st = nullptr;
-    PState* st_host = nullptr;   // pinned mirror
-    JobDev* jobs = nullptr;
-    SceneView* scenes = nullptr;
-    LdsPlan* plan = nullptr;
+    PState* st_host = nullptr;       // pinned mirror
+    uint32_t* done_host = nullptr;   // pinned mirror of job_done
     PCtx ctx{};
+    LdsPlan lds_plan{};
     struct HostJob {
         JobDev dev;
+        uint64_t work = 0;
+        uint32_t done = 0;
         bool finished = false;
     };
     std::vector<HostJob> hjobs;
@@ -452,17 +400,21 @@ struct PersistentEngine::Impl {
             if (j.dev.sbuf) (void)hipFree(j.dev.sbuf);
         free_pool();
         if (st_host) (void)hipHostFree(st_host);
+        if (done_host) (void)hipHostFree(done_host);
     }
+};
+
+struct PersistentEngine::LaunchPlan {
+    PCtx c;
+    bool single;
+    size_t lds_bytes;
+    int gx, gs, gw;
 };
 
 PersistentEngine::PersistentEngine() : m_(std::make_unique<Impl>()) {}
 PersistentEngine::~PersistentEngine() = default;
 
 uint64_t PersistentEngine::bytes() const { return pool_bytes_ + m_->sbuf_bytes; }
-
-uint64_t PersistentEngine::job_buffer_bytes(const PersistentJob& job) const {
-    return (uint64_t)job.scene->width * job.scene->height * (uint64_t)job.spp * sizeof(float4);
-}
 
 void PersistentEngine::ensure_pool(const PersistentTuning& t) {
     Impl& m = *m_;
@@ -475,12 +427,10 @@ void PersistentEngine::ensure_pool(const PersistentTuning& t) {
     cap = (cap + 1023) & ~1023;
     m.block = t.block_size > 0 ? t.block_size : kDefaultBlock;
     if (m.block != 128 && m.block != 256 && m.block != 512) throw std::runtime_error("persistent: block size must be 128, 256 or 512");
-    const int tile = t.scan_tile > 0 ? t.scan_tile : 2048;
-    m.items = tile / kRetireThreads;
-    if (m.items != 4 && m.items != 8 && m.items != 16) throw std::runtime_error("persistent: scan tile must be 1024, 2048 or 4096");
     m.graph_iters = t.graph_iters > 0 ? ((t.graph_iters + 1) & ~1) : kDefaultGraphIters;
-    m.lds_nodes = t.lds_nodes >= 0 ? t.lds_nodes : kDefaultLdsNodes;
-    m.tune = t;
+    const int kib = t.lds_kib > 0 ? t.lds_kib : kDefaultLdsKiB;
+    if (kib > 64) throw std::runtime_error("persistent: at most 64 KiB of LDS per workgroup");
+    m.lds_bytes = kib * 1024;
     if (cap == cap_ && !m.pool_allocs.empty()) return;
     m.free_graph();
     m.free_pool();
@@ -501,27 +451,23 @@ void PersistentEngine::ensure_pool(const PersistentTuning& t) {
     alloc(&c.P.T, P * 16);
     alloc(&c.P.L, P * 16);
     alloc(&c.P.key, P * 8);
-    alloc(&c.P.sp, P * 16);
-    alloc(&c.P.sn, P * 16);
+    alloc(&c.P.hit, P * 8);
     alloc(&c.SQ.o, P * 16);
     alloc(&c.SQ.d, P * 16);
     alloc(&c.SQ.c, P * 16);
     alloc(&c.wk, P * 8);
-    alloc(&c.alive, P * 4);
     alloc(&c.A[0], P * 4);
     alloc(&c.A[1], P * 4);
     alloc(&c.F, P * 4);
-    for (int q = 0; q < 3; ++q) alloc(&c.Q[q], P * 4);
-    alloc(&c.partial, (P / 1024 + 2) * 4);
-    alloc(&m.st, sizeof(PState));
-    alloc(&m.jobs, sizeof(JobDev) * kMaxJobs);
-    alloc(&m.scenes, sizeof(SceneView) * kMaxScenes);
-    alloc(&m.plan, sizeof(LdsPlan));
+    for (int q = 0; q < 4; ++q) alloc(&c.Q[q], P * 4);
+    alloc(&c.st, sizeof(PState));
+    alloc(&c.jobs, sizeof(JobDev) * kMaxJobs);
+    alloc(&c.job_done, sizeof(uint32_t) * kMaxJobs);
+    alloc(&c.scenes, sizeof(SceneView) * kMaxScenes);
+    alloc(&c.plan, sizeof(LdsPlan));
+    m.st = c.st;
     if (!m.st_host) check(hipHostMalloc(&m.st_host, sizeof(PState), hipHostMallocDefault), "hipHostMalloc");
-    c.st = m.st;
-    c.jobs = m.jobs;
-    c.scenes = m.scenes;
-    c.plan = m.plan;
+    if (!m.done_host) check(hipHostMalloc(&m.done_host, sizeof(uint32_t) * kMaxJobs, hipHostMallocDefault), "hipHostMalloc");
     c.cap = (uint32_t)cap;
 }
 
@@ -547,8 +493,33 @@ void PersistentEngine::reset_jobs() {
     m.free_graph();
     pw_init_kernel<<<(cap_ + 255) / 256, 256, 0, m.stream>>>(m.st, m.ctx.F, (uint32_t)cap_);
     check(hipGetLastError(), "persistent init");
-    LdsPlan empty{};
-    check(hipMemcpyAsync(m.plan, &empty, sizeof(empty), hipMemcpyHostToDevice, m.stream), "copy lds plan");
+    update_lds_plan();
+}
+
+// Splits the LDS budget evenly over the scenes: BVH nodes first (the breadth-first top), then the
+// scene's primitive records if all of them fit in what is left of its share.
+void PersistentEngine::update_lds_plan() {
+    Impl& m = *m_;
+    LdsPlan p{};
+    p.n = (int)m.hscenes.size();
+    if (p.n > 0) {
+        const int share = m.lds_bytes / p.n;
+        int nodes = 0, prims = 0;
+        for (int k = 0; k < p.n; ++k) {
+            const SceneView& s = m.hscenes[k];
+            p.node_cnt[k] = std::min(s.n_nodes, share / (int)sizeof(Node));
+            const int left = share - p.node_cnt[k] * (int)sizeof(Node);
+            p.prim_cnt[k] = (p.node_cnt[k] == s.n_nodes && s.n_prims * 16 <= left) ? s.n_prims : 0;
+            p.node_off[k] = nodes;
+            p.prim_off[k] = prims;
+            nodes += p.node_cnt[k];
+            prims += p.prim_cnt[k];
+        }
+        p.total_nodes = nodes;
+        p.total_prims = prims;
+    }
+    m.lds_plan = p;
+    check(hipMemcpyAsync(m.ctx.plan, &m.lds_plan, sizeof(LdsPlan), hipMemcpyHostToDevice, m.stream), "copy lds plan");
 }
 
 int PersistentEngine::admit(const PersistentJob& job) {
@@ -567,18 +538,10 @@ int PersistentEngine::admit(const PersistentJob& job) {
         sc = (int)m.scene_keys.size();
         m.scene_keys.push_back(job.scene);
         m.hscenes.push_back(*job.scene);
-        check(hipMemcpyAsync(m.scenes + sc, job.scene, sizeof(SceneView), hipMemcpyHostToDevice, m.stream), "copy scene");
-        LdsPlan plan{};
-        plan.n = (int)m.hscenes.size();
-        const int share = m.lds_nodes / plan.n;
-        int off = 0;
-        for (int k = 0; k < plan.n; ++k) {
-            plan.off[k] = off;
-            plan.cnt[k] = std::min(share, m.hscenes[k].n_nodes);
-            off += plan.cnt[k];
-        }
-        check(hipMemcpyAsync(m.plan, &plan, sizeof(plan), hipMemcpyHostToDevice, m.stream), "copy lds plan");
-        m.free_graph();   // the single/multi choice depends on the scene count
+        check(hipMemcpyAsync(m.ctx.scenes + sc, job.scene, sizeof(SceneView), hipMemcpyHostToDevice, m.stream),
+              "copy scene");
+        update_lds_plan();
+        m.free_graph();
     }
     JobDev d{};
     d.scene = sc;
@@ -595,44 +558,38 @@ int PersistentEngine::admit(const PersistentJob& job) {
     m.sbuf_bytes += (uint64_t)work * sizeof(float4);
     m.total_work = d.work_end;
     const uint32_t index = (uint32_t)m.hjobs.size();
-    pw_admit_kernel<<<1, 1, 0, m.stream>>>(m.st, m.jobs, d, index);
+    pw_admit_kernel<<<1, 1, 0, m.stream>>>(m.st, m.ctx.jobs, m.ctx.job_done, d, index);
     check(hipGetLastError(), "persistent admit");
-    m.hjobs.push_back(Impl::HostJob{d, false});
-    if (m.graph && m.graph_single) m.free_graph();   // single-job graph bakes the job in
+    Impl::HostJob hj;
+    hj.dev = d;
+    hj.work = work;
+    m.hjobs.push_back(hj);
+    if (m.graph && m.graph_single) m.free_graph();   // the single-job graph bakes the job in
     return (int)index;
 }
-
-struct PersistentEngine::LaunchPlan {
-    PCtx c;
-    bool single;
-    size_t lds_bytes;
-    int gx, gs, gw, gr;
-};
 
 PersistentEngine::LaunchPlan PersistentEngine::make_plan(bool single) {
     Impl& m = *m_;
     LaunchPlan L{};
     L.single = single;
     L.c = m.ctx;
-    L.c.n_lds = 0;
+    L.c.plan0 = m.lds_plan;
     if (single) {
         L.c.scene0 = m.hscenes[0];
         L.c.job0 = m.hjobs[0].dev;
-        L.c.n_lds = std::min(m.lds_nodes, L.c.scene0.n_nodes);
     }
-    L.lds_bytes = (size_t)(single ? L.c.n_lds : m.lds_nodes) * sizeof(Node);
-    const void* fx = single ? (const void*)pw_extend_classify_kernel<true> : (const void*)pw_extend_classify_kernel<false>;
+    L.lds_bytes = (size_t)m.lds_plan.total_nodes * sizeof(Node) + (size_t)m.lds_plan.total_prims * sizeof(float4);
+    const void* fx = single ? (const void*)pw_extend_kernel<true> : (const void*)pw_extend_kernel<false>;
     const void* fs = single ? (const void*)pw_shade_kernel<true> : (const void*)pw_shade_kernel<false>;
     const void* fw = single ? (const void*)pw_shadow_kernel<true> : (const void*)pw_shadow_kernel<false>;
     L.gx = grid_for(fx, m.block, L.lds_bytes, m.cus);
     L.gs = grid_for(fs, m.block, 0, m.cus);
     L.gw = grid_for(fw, m.block, L.lds_bytes, m.cus);
-    L.gr = m.cus * 4;
     return L;
 }
 
-// One iteration; with `ev` (6 events) the stage boundaries are recorded for profiling:
-// ev[0] plan ev[1] extend_classify ev[2] shade ev[3] shadow ev[4] retire ev[5].
+// One iteration; with `ev` (5 events) the stage boundaries are recorded for profiling:
+// ev[0] plan ev[1] extend ev[2] shade ev[3] shadow ev[4].
 void PersistentEngine::launch_iteration(const LaunchPlan& L, int par, hipEvent_t* ev) {
     Impl& m = *m_;
     hipStream_t st = m.stream;
@@ -640,36 +597,15 @@ void PersistentEngine::launch_iteration(const LaunchPlan& L, int par, hipEvent_t
     if (ev) check(hipEventRecord(ev[0], st), "hipEventRecord");
     pw_plan_kernel<<<1, 1, 0, st>>>(m.st, m.ctr);
     if (ev) check(hipEventRecord(ev[1], st), "hipEventRecord");
-    if (L.single) pw_extend_classify_kernel<true><<<L.gx, m.block, L.lds_bytes, st>>>(c, par);
-    else pw_extend_classify_kernel<false><<<L.gx, m.block, L.lds_bytes, st>>>(c, par);
+    if (L.single) pw_extend_kernel<true><<<L.gx, m.block, L.lds_bytes, st>>>(c, par);
+    else pw_extend_kernel<false><<<L.gx, m.block, L.lds_bytes, st>>>(c, par);
     if (ev) check(hipEventRecord(ev[2], st), "hipEventRecord");
-    if (L.single) pw_shade_kernel<true><<<L.gs, m.block, 0, st>>>(c);
-    else pw_shade_kernel<false><<<L.gs, m.block, 0, st>>>(c);
+    if (L.single) pw_shade_kernel<true><<<L.gs, m.block, 0, st>>>(c, par);
+    else pw_shade_kernel<false><<<L.gs, m.block, 0, st>>>(c, par);
     if (ev) check(hipEventRecord(ev[3], st), "hipEventRecord");
     if (L.single) pw_shadow_kernel<true><<<L.gw, m.block, L.lds_bytes, st>>>(c);
     else pw_shadow_kernel<false><<<L.gw, m.block, L.lds_bytes, st>>>(c);
     if (ev) check(hipEventRecord(ev[4], st), "hipEventRecord");
-    switch (m.items) {
-        case 4:
-            pw_retire_reduce_kernel<4><<<L.gr, kRetireThreads, 0, st>>>(c, par);
-            pw_retire_scan_kernel<4><<<1, 1024, 0, st>>>(c);
-            if (L.single) pw_retire_scatter_kernel<true, 4><<<L.gr, kRetireThreads, 0, st>>>(c, par);
-            else pw_retire_scatter_kernel<false, 4><<<L.gr, kRetireThreads, 0, st>>>(c, par);
-            break;
-        case 16:
-            pw_retire_reduce_kernel<16><<<L.gr, kRetireThreads, 0, st>>>(c, par);
-            pw_retire_scan_kernel<16><<<1, 1024, 0, st>>>(c);
-            if (L.single) pw_retire_scatter_kernel<true, 16><<<L.gr, kRetireThreads, 0, st>>>(c, par);
-            else pw_retire_scatter_kernel<false, 16><<<L.gr, kRetireThreads, 0, st>>>(c, par);
-            break;
-        default:
-            pw_retire_reduce_kernel<8><<<L.gr, kRetireThreads, 0, st>>>(c, par);
-            pw_retire_scan_kernel<8><<<1, 1024, 0, st>>>(c);
-            if (L.single) pw_retire_scatter_kernel<true, 8><<<L.gr, kRetireThreads, 0, st>>>(c, par);
-            else pw_retire_scatter_kernel<false, 8><<<L.gr, kRetireThreads, 0, st>>>(c, par);
-            break;
-    }
-    if (ev) check(hipEventRecord(ev[5], st), "hipEventRecord");
 }
 
 void PersistentEngine::capture(bool single) {
@@ -678,48 +614,11 @@ void PersistentEngine::capture(bool single) {
     const LaunchPlan L = make_plan(single);
     check(hipStreamBeginCapture(m.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     for (int it = 0; it < m.graph_iters; ++it) launch_iteration(L, it & 1, nullptr);
-    hipError_t e = hipStreamEndCapture(m.stream, &m.graph);
-    check(e, "hipStreamEndCapture");
+    check(hipStreamEndCapture(m.stream, &m.graph), "hipStreamEndCapture");
     check(hipGetLastError(), "persistent capture");
     check(hipGraphInstantiate(&m.exec, m.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
     m.graph_single = single;
     m.graph_scenes = (int)m.hscenes.size();
-}
-
-// Eager iterations with HIP events around each stage (synchronising once per iteration), until
-// the admitted work is done. Items per stage come from the control block of each iteration.
-void PersistentEngine::run_profiled(StageTimes* out) {
-    Impl& m = *m_;
-    if (m.hjobs.empty()) return;
-    const bool single = m.hjobs.size() == 1 && m.hscenes.size() == 1;
-    const LaunchPlan L = make_plan(single);
-    hipEvent_t ev[6];
-    for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
-    for (int it = 0;; ++it) {
-        launch_iteration(L, it & 1, ev);
-        check(hipMemcpyAsync(m.st_host, m.st, sizeof(PState), hipMemcpyDeviceToHost, m.stream), "copy state");
-        check(hipStreamSynchronize(m.stream), "profile sync");
-        const PState& s = *m.st_host;
-        float ms[5];
-        for (int k = 0; k < 5; ++k) check(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]), "hipEventElapsedTime");
-        if (s.done) break;
-        out->other += ms[0];
-        out->extend += ms[1];
-        out->shade_lambert += ms[2];
-        out->shadow += ms[3];
-        out->compact += ms[4];
-        out->items[0] += s.it_act;
-        out->items[2] += (long long)s.q[0] + s.q[1] + s.q[2];
-        out->items[5] += s.q[3];
-        out->items[6] += s.it_act;
-        out->items[7] += 1;
-        out->launches[0] += 1;
-        out->launches[2] += 1;
-        out->launches[5] += 1;
-        out->launches[6] += 3;
-        out->launches[7] += 1;
-    }
-    for (auto& e : ev) (void)hipEventDestroy(e);
 }
 
 void PersistentEngine::run(int launches) {
@@ -730,25 +629,60 @@ void PersistentEngine::run(int launches) {
     for (int k = 0; k < launches; ++k) check(hipGraphLaunch(m.exec, m.stream), "hipGraphLaunch");
 }
 
+// Eager iterations with HIP events around each stage (synchronising once per iteration), until
+// the admitted work is done. Items per stage come from the control block of each iteration.
+void PersistentEngine::run_profiled(StageTimes* out) {
+    Impl& m = *m_;
+    if (m.hjobs.empty()) return;
+    const bool single = m.hjobs.size() == 1 && m.hscenes.size() == 1;
+    const LaunchPlan L = make_plan(single);
+    hipEvent_t ev[5];
+    for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+    for (int it = 0;; ++it) {
+        launch_iteration(L, it & 1, ev);
+        check(hipMemcpyAsync(m.st_host, m.st, sizeof(PState), hipMemcpyDeviceToHost, m.stream), "copy state");
+        check(hipStreamSynchronize(m.stream), "profile sync");
+        const PState& s = *m.st_host;
+        float ms[4];
+        for (int k = 0; k < 4; ++k) check(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]), "hipEventElapsedTime");
+        if (s.done) break;
+        out->other += ms[0];
+        out->extend += ms[1];
+        out->shade_lambert += ms[2];
+        out->shadow += ms[3];
+        out->items[0] += s.it_act;
+        out->items[2] += (long long)s.q[0] + s.q[1] + s.q[2] + s.q[3];
+        out->items[5] += s.q[4];
+        out->items[7] += 1;
+        out->launches[0] += 1;
+        out->launches[2] += 1;
+        out->launches[5] += 1;
+        out->launches[7] += 1;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    // Fold the last iteration's counts (the plan of the final, empty iteration did so on device).
+}
+
 PersistentProgress PersistentEngine::progress() {
     Impl& m = *m_;
+    const size_t nj = m.hjobs.size();
     check(hipMemcpyAsync(m.st_host, m.st, sizeof(PState), hipMemcpyDeviceToHost, m.stream), "copy progress");
+    if (nj) check(hipMemcpyAsync(m.done_host, m.ctx.job_done, nj * sizeof(uint32_t), hipMemcpyDeviceToHost, m.stream),
+                  "copy job progress");
     check(hipStreamSynchronize(m.stream), "persistent sync");
+    for (size_t k = 0; k < nj; ++k) m.hjobs[k].done = m.done_host[k];
     PersistentProgress p;
     p.next_work = m.st_host->next_work;
     p.total_work = m.st_host->total_work;
-    p.min_live_work = m.st_host->min_live_work;
     p.n_live = m.st_host->n_live;
     p.done = m.st_host->done;
     p.iterations = m.st_host->iterations;
     return p;
 }
 
-bool PersistentEngine::job_complete(int job, const PersistentProgress& pr) const {
-    const JobDev& d = m_->hjobs[job].dev;
-    if (pr.next_work < d.work_end) return false;
-    if (pr.done) return true;
-    return pr.min_live_work >= d.work_end;
+bool PersistentEngine::job_complete(int job) const {
+    const Impl::HostJob& j = m_->hjobs[job];
+    return (uint64_t)j.done == j.work;
 }
 
 void PersistentEngine::finish_job(int job) {
@@ -768,7 +702,6 @@ void PersistentEngine::finish_job(int job) {
 void PersistentEngine::render(const PersistentJob* jobs, int n_jobs, const PersistentTuning& t, DevCounters* ctr,
                               hipStream_t stream, PersistentResult* out, StageTimes* stages) {
     PersistentResult res;
-    // Split jobs into consecutive sample chunks so that one chunk's sample buffers fit the budget.
     std::vector<PersistentJob> rest(jobs, jobs + n_jobs);
     uint64_t total = 0;
     for (PersistentJob& j : rest) {
@@ -780,6 +713,7 @@ void PersistentEngine::render(const PersistentJob* jobs, int n_jobs, const Persi
     const uint64_t want = tt.pool_paths > 0 ? (uint64_t)tt.pool_paths : (uint64_t)kDefaultPool;
     tt.pool_paths = (int)std::min<uint64_t>(want, std::max<uint64_t>(1024, (total + 1023) & ~1023ull));
     open(tt, ctr, stream);
+    // Split jobs into consecutive sample chunks so that one chunk's sample buffers fit the budget.
     for (;;) {
         std::vector<PersistentJob> chunk;
         uint64_t budget = sbuf_budget_;
@@ -812,7 +746,10 @@ void PersistentEngine::render(const PersistentJob* jobs, int n_jobs, const Persi
             }
         }
         res.iterations += (int)pr.iterations;
-        for (int k = 0; k < (int)m_->hjobs.size(); ++k) finish_job(k);
+        for (int k = 0; k < (int)m_->hjobs.size(); ++k) {
+            if (!job_complete(k)) throw std::runtime_error("persistent: job incomplete after the work drained");
+            finish_job(k);
+        }
         res.chunks += 1;
     }
     reset_jobs();
@@ -833,20 +770,20 @@ PersistentTuning autotune_persistent(const PersistentJob* jobs, int n_jobs, hipS
     double best_ms = 1e30;
     const int blocks[] = {256, 512};
     const int pools[] = {1 << 20, 1 << 21, 1 << 22};
-    const int tiles[] = {2048, 4096};
+    const int ldss[] = {32, 64};
     PersistentEngine eng;
     hipEvent_t a, b;
     check(hipEventCreate(&a), "hipEventCreate");
     check(hipEventCreate(&b), "hipEventCreate");
-    // Probe runs add into the jobs' films; the caller resets them afterwards.
+    // Probe renders add into the jobs' films; the caller resets them afterwards.
     for (int bs : blocks)
         for (int pool : pools) {
             if ((uint64_t)pool > 2 * work && pool != pools[0]) continue;
-            for (int tile : tiles) {
+            for (int kib : ldss) {
                 PersistentTuning t;
                 t.block_size = bs;
                 t.pool_paths = pool;
-                t.scan_tile = tile;
+                t.lds_kib = kib;
                 PersistentResult r;
                 eng.render(probe.data(), n_jobs, t, ctr, stream, &r);   // warm-up (graph capture)
                 check(hipEventRecord(a, stream), "hipEventRecord");

@@ -23,7 +23,8 @@ struct RenderParams {
     // Persistent engine tuning (0 = default / autotuned).
     int block_size = 0;
     int pool_paths = 0;
-    int scan_tile = 0;
+    int scan_tile = 0;           // plain wavefront compaction tile
+    int lds_kib = 0;             // persistent: LDS per workgroup (KiB)
     int lds_nodes = -1;          // nodes staged in LDS per workgroup (-1 = default)
 };
 

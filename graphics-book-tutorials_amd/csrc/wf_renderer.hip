@@ -144,8 +144,7 @@ void GpuRenderer::run_passes(int first_sample, int count, const RenderParams& p,
             gpu::PersistentTuning tune;
             tune.block_size = p.block_size;
             tune.pool_paths = p.pool_paths;
-            tune.scan_tile = p.scan_tile;
-            tune.lds_nodes = p.lds_nodes;
+            tune.lds_kib = p.lds_kib;
             gpu::PersistentResult pr;
             m.pe->render(&job, 1, tune, m.d_ctr, m.stream, &pr, p.stage_timing ? &st : nullptr);
             iterations = pr.iterations;

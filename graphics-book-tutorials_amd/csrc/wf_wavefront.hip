@@ -44,7 +44,7 @@ __global__ __launch_bounds__(kThreads) void wf_extend_kernel(SceneView s, PathPo
     r.tmin = kRayEps;
     r.tmax = kInf;
     Hit h;
-    intersect_with(s, LdsNodes{lds_nodes, s.nodes, n_lds}, r, &h, false);
+    intersect_with(s, lds_nodes_of(s, lds_nodes, n_lds), r, &h, false);
     P.hit[i] = make_float2(h.t, __int_as_float(h.prim));
 }
 
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kThreads) void wf_shadow_kernel(SceneView s, PathPo
     r.tmin = kRayEps;
     r.tmax = o.w;
     Hit h;
-    if (!intersect_with(s, LdsNodes{lds_nodes, s.nodes, n_lds}, r, &h, true)) {
+    if (!intersect_with(s, lds_nodes_of(s, lds_nodes, n_lds), r, &h, true)) {
         const uint32_t i = (uint32_t)__float_as_int(d.w);
         const float4 c = SQ.c[idx];
         float4 l = P.L[i];

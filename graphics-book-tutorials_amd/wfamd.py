@@ -33,7 +33,7 @@ class RenderParams(C.Structure):
     _fields_ = [("spp", C.c_int), ("seed", C.c_uint32), ("max_depth", C.c_int), ("threads", C.c_int),
                 ("count_rays", C.c_int), ("stage_timing", C.c_int), ("queue_capacity", C.c_int),
                 ("trace_bounces", C.c_int), ("block_size", C.c_int), ("pool_paths", C.c_int),
-                ("scan_tile", C.c_int), ("lds_nodes", C.c_int)]
+                ("scan_tile", C.c_int), ("lds_nodes", C.c_int), ("lds_kib", C.c_int)]
 
 
 class RenderStats(C.Structure):
@@ -90,7 +90,7 @@ class BatchStats(C.Structure):
     _fields_ = [("render_ms", C.c_double), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("invalid_hits", C.c_int64), ("queue_overflows", C.c_int64), ("iterations", C.c_int64),
                 ("graph_launches", C.c_int64), ("block_size", C.c_int), ("pool_paths", C.c_int),
-                ("scan_tile", C.c_int), ("device_bytes", C.c_uint64)]
+                ("lds_kib", C.c_int), ("device_bytes", C.c_uint64)]
 
 
 class ServerStats(C.Structure):
@@ -189,9 +189,9 @@ def _iptr(a):
 
 
 def params(spp=64, seed=1234, max_depth=8, threads=0, stage_timing=False, queue_capacity=0, trace_bounces=False,
-           block_size=0, pool_paths=0, scan_tile=0, lds_nodes=-1):
+           block_size=0, pool_paths=0, scan_tile=0, lds_nodes=-1, lds_kib=0):
     return RenderParams(spp, seed, max_depth, threads, 0, int(stage_timing), queue_capacity, int(trace_bounces),
-                        block_size, pool_paths, scan_tile, lds_nodes)
+                        block_size, pool_paths, scan_tile, lds_nodes, lds_kib)
 
 
 class Scene:

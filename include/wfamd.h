@@ -49,8 +49,9 @@ typedef struct wf_render_params {
     int trace_bounces;    /* wavefront: record per-bounce integer work, see wf_renderer_trace */
     int block_size;       /* persistent: threads per workgroup (128, 256, 512) */
     int pool_paths;       /* persistent: path slots (= queue capacity) */
-    int scan_tile;        /* wavefront/persistent: compaction tile (elements) */
-    int lds_nodes;        /* BVH nodes staged in LDS per workgroup, -1 = default */
+    int scan_tile;        /* plain wavefront: compaction tile (elements) */
+    int lds_nodes;        /* megakernel / plain wavefront: BVH nodes staged in LDS, -1 = default */
+    int lds_kib;          /* persistent: LDS per workgroup for BVH nodes + primitive records (KiB) */
 } wf_render_params;
 
 /* RenderStats, include/wf/renderer.hpp:23-37. stage_ms: extend, classify, shade_lambert,
@@ -202,13 +203,13 @@ typedef struct wf_batch_stats {
     uint64_t extension_rays, shadow_rays;
     int64_t invalid_hits, queue_overflows;
     int64_t iterations, graph_launches;
-    int block_size, pool_paths, scan_tile;
+    int block_size, pool_paths, lds_kib;
     uint64_t device_bytes;
 } wf_batch_stats;
 
 int wf_batch_render(const wf_job* jobs, int n_jobs, const wf_render_params* tuning, wf_batch_stats* stats);
 /* Autotuner: times short renders of the batch (probe_spp samples per job) over block size, pool
- * size and scan tile; writes the fastest into *tuning. */
+ * size (= queue capacity) and LDS budget; writes the fastest into *tuning. */
 int wf_autotune(const wf_job* jobs, int n_jobs, int probe_spp, wf_render_params* tuning);
 
 /* ---- serving loop: continuous batching under a device-memory budget (no reference counterpart) ---- */
