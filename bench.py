@@ -25,14 +25,6 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 METRIC = "Mrays/s at 1280x720x64spp depth 8 per scene; p50/p99 render latency"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic HBM bytes per item of each persistent-engine kernel (device layout of
-# graphics-book-tutorials_amd/csrc/wf_pathpool.h and wf_persistent.hip):
-#   extend per live path: active index 4, ray 32, hit 8, queue entry 4 (BVH and primitives in LDS/L2)
-#   shade per queue entry: entry 4, work id 8, ray + throughput + radiance + key 72, hit 8, then
-#         ray + throughput 48 and radiance (or the sample) 16, next-active or free entry 4,
-#         and 48 per shadow ray pushed (counted here at the Lambert share, ~0.6 x 48 ~ 28)
-#   shadow per shadow ray: entry 48, radiance read-modify-write 32
-BYTES_PER_ITEM = {"extend": 48, "shade": 188, "shadow": 80}
 
 
 def percentile(v, p):
@@ -77,6 +69,7 @@ def main():
     ap.add_argument("--height", type=int, default=720)
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-spp", type=int, default=16, help="samples per pixel of the CPU baseline sample")
     ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -97,7 +90,9 @@ def main():
         t0 = time.perf_counter()
         tuned = wfamd.autotune([(scene, spp, seed, depth)], probe_spp=4)
         tuning.block_size, tuning.pool_paths, tuning.lds_kib = tuned.block_size, tuned.pool_paths, tuned.lds_kib
+        tuning.refill = tuned.refill
         print(f"[bench] autotune: block {tuned.block_size} pool {tuned.pool_paths} lds {tuned.lds_kib} KiB "
+              f"refill {tuned.refill} "
               f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
 
     r = wfamd.Renderer(scene, "persistent")
@@ -137,10 +132,11 @@ def main():
           file=sys.stderr, flush=True)
 
     # Stage-timed render (eager launches, HIP events on the engine's stream around each stage):
-    # per-kernel average launch durations and the items each launch processed.
+    # per-kernel average launch durations, items and algorithmic device-memory bytes per launch
+    # (computed by the engine from each iteration's counts; layouts in wf_persistent.hip).
     roof = None
     prof = wfamd.params(spp=spp, seed=seed, max_depth=depth, stage_timing=True, block_size=tuning.block_size,
-                        pool_paths=tuning.pool_paths, lds_kib=tuning.lds_kib)
+                        pool_paths=tuning.pool_paths, lds_kib=tuning.lds_kib, refill=tuning.refill)
     r.reset()
     pst = r.run_passes(0, spp, prof)
     names = {"extend": "extend", "shade_lambert": "shade", "shadow": "shadow"}
@@ -149,17 +145,19 @@ def main():
     if stages:
         dom = max(stages, key=lambda k: stages[k]["ms"])
         d = stages[dom]
-        bytes_per_launch = BYTES_PER_ITEM[dom] * d["items"] / max(d["launches"], 1)
-        avg_s = d["ms"] / 1e3 / max(d["launches"], 1)
-        achieved = bytes_per_launch / avg_s / 1e9
+        launches = max(d["launches"], 1)
+        avg_s = d["ms"] / 1e3 / launches
+        achieved = d["bytes"] / launches / avg_s / 1e9
         roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "avg_launch_us": avg_s * 1e6, "bytes_per_item": BYTES_PER_ITEM[dom],
-                "items_per_launch": d["items"] / max(d["launches"], 1),
-                "stages_ms": {k: v["ms"] for k, v in stages.items()}}
+                "avg_launch_us": avg_s * 1e6, "bytes_per_launch": d["bytes"] / launches,
+                "items_per_launch": d["items"] / launches,
+                "stages": {k: {"ms": v["ms"], "launches": v["launches"], "items": v["items"],
+                               "gbs": v["bytes"] / (v["ms"] / 1e3) / 1e9 if v["ms"] > 0 else None}
+                           for k, v in stages.items()}}
     r.close()
 
-    cpu = None if args.no_cpu else cpu_baseline(wfamd, args.scene, args.width, args.height, 4)
+    cpu = None if args.no_cpu else cpu_baseline(wfamd, args.scene, args.width, args.height, args.cpu_spp)
     value = (ext + sh) / elapsed / 1e6
     out = {
         "metric": METRIC, "value": value, "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -169,6 +167,7 @@ def main():
         "config": {"workload": f"{args.scene} {args.width}x{args.height} {spp}spp depth {depth} seed {seed} "
                                f"(material-divergent scene)", "arm": "persistent batched wavefront",
                    "block_size": tuning.block_size, "pool_paths": tuning.pool_paths, "lds_kib": tuning.lds_kib,
+                   "refill": tuning.refill,
                    "parallelism": "single GPU"},
         "p50_ms": percentile(lat, 0.5), "p99_ms": percentile(lat, 0.99),
         "extension_mrays_per_s": ext / elapsed / 1e6, "rays_per_render": rays_per_render,

@@ -63,6 +63,7 @@ wf::RenderParams to_params(const wf_render_params* p) {
     r.scan_tile = p->scan_tile;
     r.lds_nodes = p->lds_nodes;
     r.lds_kib = p->lds_kib;
+    r.refill = p->refill;
     return r;
 }
 
@@ -94,6 +95,7 @@ void to_stats(const wf::RenderStats& s, wf_render_stats* o) {
     for (int k = 0; k < 8; ++k) {
         o->stage_items[k] = s.stages.items[k];
         o->stage_launches[k] = s.stages.launches[k];
+        o->stage_bytes[k] = s.stages.bytes[k];
     }
 }
 
@@ -605,6 +607,7 @@ wf::gpu::PersistentTuning to_tuning(const wf_render_params* p) {
         t.block_size = p->block_size;
         t.pool_paths = p->pool_paths;
         t.lds_kib = p->lds_kib;
+        t.refill = p->refill;
     }
     return t;
 }
@@ -679,6 +682,7 @@ extern "C" int wf_batch_render(const wf_job* jobs, int n_jobs, const wf_render_p
             stats->block_size = t.block_size;
             stats->pool_paths = eng.pool_paths();
             stats->lds_kib = t.lds_kib;
+            stats->refill = t.refill;
             stats->device_bytes = eng.bytes() + b.scenes.bytes();
         }
         return (int)WF_OK;
@@ -694,6 +698,7 @@ extern "C" int wf_autotune(const wf_job* jobs, int n_jobs, int probe_spp, wf_ren
         tuning->block_size = best.block_size;
         tuning->pool_paths = best.pool_paths;
         tuning->lds_kib = best.lds_kib;
+        tuning->refill = best.refill;
         return (int)WF_OK;
     });
 }

@@ -299,24 +299,28 @@ WF_HD bool hit_triangle(Vec3 p0, Vec3 p1, Vec3 p2, const Ray& r, float* t_out) {
 }
 
 // Slab test against [tmin, tmax].
+// The reference returns early once t0 > t1 after an axis. This form evaluates all three axes and
+// compares once (no divergent branches on a GPU); the result is the same: after t0 > t1 the
+// remaining axes can only raise t0 or lower t1 (a NaN slab, 0 * inf, fails both comparisons and
+// changes neither), so the final t0 <= t1 is false exactly when an early exit would have been taken.
 WF_HD bool hit_box(Vec3 lo, Vec3 hi, Vec3 o, Vec3 inv_d, float tmin, float tmax) {
     float t0 = tmin, t1 = tmax;
     float tn = (lo.x - o.x) * inv_d.x, tf = (hi.x - o.x) * inv_d.x;
-    if (tn > tf) { float s = tn; tn = tf; tf = s; }
-    t0 = tn > t0 ? tn : t0;
-    t1 = tf < t1 ? tf : t1;
-    if (t0 > t1) return false;
+    float a = tn > tf ? tf : tn, b = tn > tf ? tn : tf;
+    t0 = a > t0 ? a : t0;
+    t1 = b < t1 ? b : t1;
     tn = (lo.y - o.y) * inv_d.y;
     tf = (hi.y - o.y) * inv_d.y;
-    if (tn > tf) { float s = tn; tn = tf; tf = s; }
-    t0 = tn > t0 ? tn : t0;
-    t1 = tf < t1 ? tf : t1;
-    if (t0 > t1) return false;
+    a = tn > tf ? tf : tn;
+    b = tn > tf ? tn : tf;
+    t0 = a > t0 ? a : t0;
+    t1 = b < t1 ? b : t1;
     tn = (lo.z - o.z) * inv_d.z;
     tf = (hi.z - o.z) * inv_d.z;
-    if (tn > tf) { float s = tn; tn = tf; tf = s; }
-    t0 = tn > t0 ? tn : t0;
-    t1 = tf < t1 ? tf : t1;
+    a = tn > tf ? tf : tn;
+    b = tn > tf ? tn : tf;
+    t0 = a > t0 ? a : t0;
+    t1 = b < t1 ? b : t1;
     return t0 <= t1;
 }
 
@@ -348,35 +352,65 @@ WF_HD MemNodes mem_nodes(const SceneView& s) { return MemNodes{s.nodes, s.prims,
 // child (leaf: test its primitives, then follow skip); on a miss follow skip. tmax shrinks with
 // each hit; a strictly nearer hit wins, so ties keep the earlier primitive in traversal order.
 // anyhit returns on the first hit (shadow rays).
+//
+// The traversal is written as a resumable step so that a device kernel can interleave rays in one
+// lane (a lane whose ray finished takes a new ray while the rest of its wave is still traversing)
+// and still visit exactly the nodes and primitives, in exactly the order, of the loop below.
+// Traversal state: the ray (its tmax is the closest hit distance so far), 1/d, the closest slot
+// and the next node; node -1 means finished.
+struct Traversal {
+    Ray r;
+    Vec3 inv_d;
+    int prim;
+    int node;
+};
+
+WF_HD Traversal traversal_begin(const Ray& r) {
+    Traversal tr;
+    tr.r = r;
+    tr.inv_d = Vec3{1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};
+    tr.prim = -1;
+    tr.node = 0;
+    return tr;
+}
+
+// Visits one node (a leaf with its primitives); sets node = -1 once the traversal is over.
 template <class NodeSrc>
-WF_HD bool intersect_with(const SceneView& s, const NodeSrc& src, Ray r, Hit* h, bool anyhit) {
-    h->t = r.tmax;
-    h->prim = -1;
-    Vec3 inv_d{1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};
-    int n = 0;
-    while (n >= 0 && n < s.n_nodes) {
-        const Node node = src.node(n);
-        if (hit_box(node.lo, node.hi, r.o, inv_d, r.tmin, r.tmax)) {
-            if (node.child < 0) {
-                const int code = -(node.child + 1);
-                const int first = code >> 2, cnt = (code & 3) + 1;
-                for (int i = 0; i < cnt; ++i) {
-                    float t;
-                    if (src.hit(first + i, r, &t)) {
-                        r.tmax = t;
-                        h->t = t;
-                        h->prim = first + i;
-                        if (anyhit) return true;
+WF_HD void traversal_step(int n_nodes, const NodeSrc& src, Traversal& tr, bool anyhit) {
+    const Node node = src.node(tr.node);
+    int n;
+    if (hit_box(node.lo, node.hi, tr.r.o, tr.inv_d, tr.r.tmin, tr.r.tmax)) {
+        if (node.child < 0) {
+            const int code = -(node.child + 1);
+            const int first = code >> 2, cnt = (code & 3) + 1;
+            for (int i = 0; i < cnt; ++i) {
+                float t;
+                if (src.hit(first + i, tr.r, &t)) {
+                    tr.r.tmax = t;
+                    tr.prim = first + i;
+                    if (anyhit) {
+                        tr.node = -1;
+                        return;
                     }
                 }
-                n = node.skip;
-            } else {
-                n = node.child;
             }
-        } else {
             n = node.skip;
+        } else {
+            n = node.child;
         }
+    } else {
+        n = node.skip;
     }
+    tr.node = (n >= 0 && n < n_nodes) ? n : -1;
+}
+
+template <class NodeSrc>
+WF_HD bool intersect_with(const SceneView& s, const NodeSrc& src, Ray r, Hit* h, bool anyhit) {
+    Traversal tr = traversal_begin(r);
+    if (s.n_nodes <= 0) tr.node = -1;
+    while (tr.node >= 0) traversal_step(s.n_nodes, src, tr, anyhit);
+    h->t = tr.r.tmax;
+    h->prim = tr.prim;
     return h->prim >= 0;
 }
 

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -104,34 +105,81 @@ __device__ __forceinline__ void wave_add_u32(unsigned int* dst, uint32_t v) {
     if (lane_id() == 0 && s) atomicAdd(dst, s);
 }
 
+// Address-space-qualified pointers to a native 4-float vector: loads through lds_float4 are
+// ds_read instructions, through glb_float4 global loads (a generic pointer read from memory, such
+// as a scene table entry, would otherwise give flat loads).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) const f32x4 lds_float4;
+typedef __attribute__((address_space(1))) const f32x4 glb_float4;
+
+static_assert(sizeof(Node) == 32 && offsetof(Node, skip) == 12 && offsetof(Node, child) == 28, "Node layout");
+static_assert(sizeof(Prim) == 48 && offsetof(Prim, id) == 12 && offsetof(Prim, radius) == 28 &&
+                  offsetof(Prim, material) == 44, "Prim layout");
+
+// Node n of a device node array (32-byte records {lo, skip, hi, child}).
+__device__ __forceinline__ Node load_node_global(const Node* nodes, int n) {
+    const glb_float4* q = (const glb_float4*)nodes + 2 * n;
+    const f32x4 a = q[0], b = q[1];
+    Node out;
+    out.lo = Vec3{a.x, a.y, a.z};
+    out.skip = __float_as_int(a.w);
+    out.hi = Vec3{b.x, b.y, b.z};
+    out.child = __float_as_int(b.w);
+    return out;
+}
+
 // Scene source that serves the first n_lds nodes (the breadth-first top of the tree) from LDS and,
-// when lds_prims is set, every leaf primitive from a compact LDS record: centre and radius of a
+// when has_lds_prims is set, every leaf primitive from a compact LDS record: centre and radius of a
 // sphere, or the first vertex of a triangle with w = -1 (the other two vertices then come from
 // memory). Spheres always have a positive radius (validated), so w < 0 marks a triangle.
+// A node is one 32-byte record {lo, skip, hi, child}, read from LDS as two 16-byte loads. The
+// scheduling barrier closing the LDS branch keeps the compiler from merging the LDS and memory
+// loads into one flat load through a selected pointer (a flat load waits on both the LDS and the
+// memory counters).
 struct LdsNodes {
-    const Node* lds;
+    lds_float4* lds;
     const Node* mem;
     int n_lds;
     const Prim* prims;
     int n_spheres;
-    const float4* lds_prims = nullptr;
-    __device__ __forceinline__ Node node(int n) const { return n < n_lds ? lds[n] : mem[n]; }
-    __device__ __forceinline__ bool hit(int slot, const Ray& r, float* t) const {
-        if (lds_prims) {
-            const float4 q = lds_prims[slot];
-            if (q.w >= 0.0f) return hit_sphere(Vec3{q.x, q.y, q.z}, q.w, r, t);
-            const Prim& p = prims[slot];
-            return hit_triangle(Vec3{q.x, q.y, q.z}, p.p1, p.p2, r, t);
+    lds_float4* lds_prims;
+    int has_lds_prims;
+    __device__ __forceinline__ Node node(int n) const {
+        Node out;
+        if (n < n_lds) {
+            const f32x4 a = lds[2 * n], b = lds[2 * n + 1];
+            out.lo = Vec3{a.x, a.y, a.z};
+            out.skip = __float_as_int(a.w);
+            out.hi = Vec3{b.x, b.y, b.z};
+            out.child = __float_as_int(b.w);
+            __builtin_amdgcn_sched_barrier(0);
+        } else {
+            out = load_node_global(mem, n);
         }
-        const Prim& p = prims[slot];
-        if (p.id < n_spheres) return hit_sphere(p.p0, p.radius, r, t);
-        return hit_triangle(p.p0, p.p1, p.p2, r, t);
+        return out;
+    }
+    // Prim records are 48 bytes: {p0, id}, {p1, radius}, {p2, material}.
+    __device__ __forceinline__ bool hit(int slot, const Ray& r, float* t) const {
+        const glb_float4* g = (const glb_float4*)prims + 3 * slot;
+        if (has_lds_prims) {
+            const f32x4 q = lds_prims[slot];
+            if (q.w >= 0.0f) return hit_sphere(Vec3{q.x, q.y, q.z}, q.w, r, t);
+            const f32x4 b = g[1], c = g[2];
+            return hit_triangle(Vec3{q.x, q.y, q.z}, Vec3{b.x, b.y, b.z}, Vec3{c.x, c.y, c.z}, r, t);
+        }
+        const f32x4 a = g[0], b = g[1];
+        if (__float_as_int(a.w) < n_spheres) return hit_sphere(Vec3{a.x, a.y, a.z}, b.w, r, t);
+        const f32x4 c = g[2];
+        return hit_triangle(Vec3{a.x, a.y, a.z}, Vec3{b.x, b.y, b.z}, Vec3{c.x, c.y, c.z}, r, t);
     }
 };
 
+// `lds` and `lds_prims` must point into LDS (__shared__ memory).
 __device__ __forceinline__ LdsNodes lds_nodes_of(const SceneView& s, const Node* lds, int n_lds,
                                                  const float4* lds_prims = nullptr) {
-    return LdsNodes{lds, s.nodes, n_lds, s.prims, s.n_spheres, lds_prims};
+    return LdsNodes{(lds_float4*)lds, s.nodes, n_lds, s.prims, s.n_spheres,
+                    (lds_float4*)(lds_prims ? lds_prims : reinterpret_cast<const float4*>(lds)),
+                    lds_prims ? 1 : 0};
 }
 
 // Compact LDS record of leaf slot `slot` (see LdsNodes).

@@ -1,19 +1,24 @@
-// Persistent batched wavefront. A fixed pool of path slots is kept full: each iteration
-// regenerates freed slots from a global work counter over all admitted jobs (work item = one
-// (job, sample, pixel) path, sample-major inside a job, pixels in 8x8 tiles), then
-//   extend  traverses every live path (BVH top and, when they fit, compact primitive records in
-//           LDS), stores the hit and pushes the path into one of four queues by the material it
-//           hit: terminal (miss, emitter, invalid hit), Lambert, metal, dielectric;
-//   shade   classifies and shades the four queues as one index space (a wave sees one queue
-//           except at the boundaries); survivors are pushed to the next active list, finished
-//           paths write their radiance and push their slot to the free stack, Lambert pushes
-//           shadow rays;
-//   shadow  traces the shadow queue (any hit), adds unoccluded contributions and finishes paths
-//           that ended with a shadow ray pending.
-// All pushes are wave64-aggregated (ballot + mbcnt, one atomic per wave). Every stage kernel runs
-// on a persistent grid and reads its item count from a device control block, so an iteration
-// needs no host round trip; K iterations are captured once into a hipGraph and replayed until the
-// control block reports that all admitted work is done.
+// Persistent batched wavefront. A pool of path slots is kept full from a global work counter over
+// all admitted jobs (work item = one (job, sample, pixel) path, sample-major inside a job, pixels
+// in 8x8 tiles). The live paths of an iteration are the items [0, n_act): the survivors of the
+// previous iteration, stored densely in item order, then the new paths, which are generated from
+// their work index wherever they are needed instead of being stored. Per iteration:
+//   plan    1 thread: fold the previous iteration's counts, size this one
+//   extend  closest-hit traversal of every item (hit -> hit[item])
+//   shade   classify + shade every item (all materials in one pass); a survivor's whole state is
+//           written compacted, in item order, to the other buffer; a finished path writes its
+//           radiance to its job's sample buffer; Lambert bounces append a shadow ray
+//   shadow  any-hit traversal of the shadow rays; an unoccluded one adds its contribution to the
+//           survivor's radiance (or to the sample of a path that ended with it)
+// Survivor and shadow-ray slots are reserved once per workgroup round (the waves' ballot counts are
+// combined in LDS, one atomic per counter per round), so every state read and write of extend and
+// shade is a contiguous run. Traversal kernels run "while-while" with dynamic fetch: every wave
+// owns a contiguous range of rays and a lane whose ray has finished takes the next one as soon as
+// fewer than `refill` lanes of the wave are still traversing, so the wave does not idle on its
+// longest ray. The traversal step itself is the shared one, so hits are the megakernel's bits.
+// Every stage kernel reads its item count from the device control block, so an iteration needs
+// no host round trip; K iterations are captured once into a hipGraph and replayed until the control
+// block reports that all admitted work is done.
 //
 // A finished path writes its radiance to its job's sample buffer at (sample, pixel); when a job's
 // paths have all finished, a resolve kernel adds the samples to the film in sample order, so the
@@ -44,10 +49,11 @@ struct PersistentJob {
 };
 
 struct PersistentTuning {
-    int block_size = 0;    // threads per workgroup: 128, 256 or 512 (0 -> 512)
+    int block_size = 0;    // threads per workgroup: 256, 512 or 1024 (0 -> 512)
     int pool_paths = 0;    // path slots = queue capacity (0 -> 2^22)
     int lds_kib = 0;       // LDS per workgroup for BVH nodes and primitive records (0 -> 64 KiB)
     int graph_iters = 0;   // iterations per captured graph (0 -> 16; rounded up to even)
+    int refill = 0;        // traversal refill threshold in lanes of 64 (0 -> 32)
 };
 
 struct PersistentResult {
@@ -66,13 +72,24 @@ struct JobDev {
     DeviceFilm film;
 };
 
-// Device control block. Queues: 0 terminal, 1 Lambert, 2 metal, 3 dielectric, 4 shadow.
+// Device control block.
 struct PState {
     unsigned long long next_work, total_work, it_work_base, pad64;
-    uint32_t n_live, n_free, n_jobs, have_prev;
-    uint32_t it_live_old, it_new, it_act, it_free_after;
-    uint32_t q[5];
-    uint32_t n_next, n_freed, iterations, done, overflow;
+    uint32_t n_live, n_jobs, have_prev, iterations;
+    uint32_t it_live, it_new, it_act, done;
+    uint32_t n_next, n_shadow, pad[2];
+};
+
+// One buffer of dense per-item path state (two of them alternate by iteration).
+//   ro : origin xyz, prev_pdf      rd : direction xyz, depth | specular << 16
+//   T  : throughput xyz            L  : radiance xyz
+//   kw : PathKey, job index, sample-buffer index
+struct PathBuf {
+    float4* ro = nullptr;
+    float4* rd = nullptr;
+    float4* T = nullptr;
+    float4* L = nullptr;
+    uint4* kw = nullptr;
 };
 
 // LDS layout of a workgroup: nodes of each scene, then primitive records of each scene.

@@ -1,9 +1,9 @@
 // Persistent batched wavefront engine; the design is described in wf_persistent.h.
 // Per iteration (all counts live in the device control block PState):
-//   plan    1 thread: fold the previous iteration's results, size this one
-//   extend  regenerate freed slots, traverse every live path, push it by hit material
-//   shade   classify + shade the four queues; re-queue survivors, finish the others
-//   shadow  any-hit shadow rays; finish paths whose last bounce left a shadow ray pending
+//   plan    1 thread: fold the previous iteration's counts, size this one
+//   extend  closest-hit traversal of items [0, n_act) (survivors, then new paths)
+//   shade   classify + shade every item; compact survivors into the other buffer, finish the rest
+//   shadow  any-hit shadow rays; add unoccluded contributions
 // Stage semantics and the order of L accumulation are those of the plain wavefront
 // (src/gpu/wavefront.cu:88-270): emission in classify, then the unoccluded NEE contribution
 // before the next classify, so every path computes the megakernel's bits.
@@ -20,6 +20,9 @@ constexpr int kDefaultPool = 1 << 22;
 constexpr int kDefaultBlock = 512;
 constexpr int kDefaultGraphIters = 16;
 constexpr int kDefaultLdsKiB = 64;
+constexpr int kDefaultRefill = 32;
+constexpr uint32_t kIdle = 0xFFFFFFFFu;
+constexpr uint32_t kFinalFlag = 0x80000000u;
 
 struct PCtx {
     PState* st;
@@ -27,14 +30,11 @@ struct PCtx {
     uint32_t* job_done;   // finished paths per job
     SceneView* scenes;
     LdsPlan* plan;
-    PathPool P;
-    ShadowQueue SQ;
-    uint2* wk;            // (job, local work index) per slot
-    uint32_t* A[2];       // active lists (ping-pong by iteration parity)
-    uint32_t* F;          // free-slot stack
-    uint32_t* Q[4];       // terminal, Lambert, metal, dielectric queues
+    float2* hit;          // per item: t, leaf slot (-1 = miss)
+    ShadowQueue SQ;       // o: origin, tmax; d: direction, job | final flag; c: contribution, destination
     DevCounters* ctr;
     uint32_t cap;
+    int refill;
     LdsPlan plan0;        // single mode: the LDS layout
     SceneView scene0;     // single mode: the scene
     JobDev job0;          // single mode: the job
@@ -75,6 +75,25 @@ __device__ __forceinline__ void tile_pixel(uint32_t q, uint32_t W, uint32_t H, u
     *py = 8u * b + in / wb;
 }
 
+// The path of work index w: its job, its sample-buffer index (sample-major, then pixel) and its
+// initial state (include/wf/path_core.hpp:59-82).
+template <bool SINGLE>
+__device__ __forceinline__ PathState new_path(const PCtx& c, uint32_t n_jobs, unsigned long long w, uint32_t* jid_out,
+                                              uint32_t* sidx_out) {
+    uint32_t jid = 0;
+    if constexpr (!SINGLE) jid = (uint32_t)find_job(c.jobs, n_jobs, w);
+    const JobDev& job = job_of<SINGLE>(c, jid);
+    const SceneView& s = scene_of<SINGLE>(c, job.scene);
+    const uint32_t local = (uint32_t)(w - job.work_begin);
+    const uint32_t W = (uint32_t)job.width, WH = W * (uint32_t)job.height;
+    const uint32_t q = local / WH;
+    uint32_t px, py;
+    tile_pixel(local - q * WH, W, (uint32_t)job.height, &px, &py);
+    *jid_out = jid;
+    *sidx_out = q * WH + py * W + px;
+    return init_path(s, (int)px, (int)py, job.first_sample + q, job.seed);
+}
+
 // Stages the LDS layout (BVH tops and primitive records); the whole block calls it.
 template <bool SINGLE>
 __device__ __forceinline__ void stage_lds(const PCtx& c, Node* lds, LdsPlan* sp) {
@@ -112,14 +131,31 @@ __device__ __forceinline__ void count_done(uint32_t* job_done, bool pred, uint32
     }
 }
 
-__global__ void pw_init_kernel(PState* st, uint32_t* F, uint32_t cap) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < cap) F[i] = i;
-    if (i == 0) {
-        PState z{};
-        z.n_free = cap;
-        *st = z;
-    }
+// Items of a traversal kernel: every wave of the grid owns one contiguous range of n.
+struct WaveRange {
+    uint32_t begin, end, per;
+};
+
+__device__ __forceinline__ WaveRange wave_range(uint32_t n) {
+    const uint32_t wpb = blockDim.x >> 6;
+    const uint32_t nw = gridDim.x * wpb;
+    const uint32_t per = (n + nw - 1) / nw;
+    const uint32_t gw = blockIdx.x * wpb + (threadIdx.x >> 6);
+    WaveRange r;
+    r.per = per;
+    r.begin = min(gw * per, n);
+    r.end = min(r.begin + per, n);
+    return r;
+}
+
+// True when no wave of this block owns an item (the block returns before staging LDS).
+__device__ __forceinline__ bool block_idle(uint32_t n, uint32_t per) {
+    return blockIdx.x * (blockDim.x >> 6) * per >= n;
+}
+
+__global__ void pw_init_kernel(PState* st) {
+    PState z{};
+    *st = z;
 }
 
 __global__ void pw_admit_kernel(PState* st, JobDev* jobs, uint32_t* job_done, JobDev job, uint32_t index) {
@@ -130,205 +166,271 @@ __global__ void pw_admit_kernel(PState* st, JobDev* jobs, uint32_t* job_done, Jo
     st->done = 0;
 }
 
-__global__ void pw_plan_kernel(PState* st, DevCounters* ctr) {
+__global__ void pw_plan_kernel(PState* st, DevCounters* ctr, uint32_t cap) {
     if (st->have_prev) {
-        const uint32_t act = st->it_act;
-        ctr->extension_rays += act;
-        ctr->shadow_rays += st->q[4];
-        for (int k = 0; k < 4; ++k)
-            if (st->q[k + 1] > ctr->high_water[k]) ctr->high_water[k] = st->q[k + 1];
-        if (act > ctr->high_water[4]) ctr->high_water[4] = act;
-        ctr->overflows += st->overflow;
+        ctr->extension_rays += st->it_act;
+        ctr->shadow_rays += st->n_shadow;
+        if (st->n_shadow > ctr->high_water[3]) ctr->high_water[3] = st->n_shadow;
+        if (st->it_act > ctr->high_water[4]) ctr->high_water[4] = st->it_act;
         st->n_live = st->n_next;
-        st->n_free = st->it_free_after + st->n_freed;
     }
     st->have_prev = 1;
     const unsigned long long rem = st->total_work - st->next_work;
-    const uint32_t n_new = (uint32_t)((unsigned long long)st->n_free < rem ? (unsigned long long)st->n_free : rem);
-    st->it_live_old = st->n_live;
+    const uint32_t room = cap - st->n_live;
+    const uint32_t n_new = (uint32_t)((unsigned long long)room < rem ? (unsigned long long)room : rem);
+    st->it_live = st->n_live;
     st->it_new = n_new;
     st->it_act = st->n_live + n_new;
-    st->it_free_after = st->n_free - n_new;
     st->it_work_base = st->next_work;
     st->next_work += n_new;
-    for (int k = 0; k < 5; ++k) st->q[k] = 0;
     st->n_next = 0;
-    st->n_freed = 0;
-    st->overflow = 0;
+    st->n_shadow = 0;
     if (st->it_act > 0) st->iterations += 1;
     st->done = st->it_act == 0 ? 1u : 0u;
 }
 
+// Closest-hit traversal of items [0, it_act), "while-while" with dynamic fetch per wave.
 template <bool SINGLE>
-__global__ __launch_bounds__(512) void pw_extend_kernel(PCtx c, int parity) {
+__global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
     extern __shared__ Node lds[];
     __shared__ LdsPlan sp;
-    stage_lds<SINGLE>(c, lds, &sp);
-    PState* st = c.st;
-    const uint32_t n_act = st->it_act, n_old = st->it_live_old, free_after = st->it_free_after;
+    const PState* st = c.st;
+    const uint32_t n_act = st->it_act, n_live = st->it_live, n_jobs = st->n_jobs;
     const unsigned long long wbase = st->it_work_base;
-    const uint32_t n_jobs = st->n_jobs;
-    uint32_t* A = parity ? c.A[1] : c.A[0];
-    uint32_t inv = 0;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n_act; b0 += stride) {
-        const uint32_t idx = b0 + threadIdx.x;
-        const bool valid = idx < n_act;
-        uint32_t slot = 0;
-        int cat = -1;
-        if (valid) {
-            uint32_t jid = 0;
-            Ray r;
-            if (idx >= n_old) {
-                const uint32_t k = idx - n_old;
-                slot = c.F[free_after + k];
-                A[idx] = slot;
-                const unsigned long long w = wbase + k;
-                if constexpr (!SINGLE) jid = (uint32_t)find_job(c.jobs, n_jobs, w);
-                const JobDev& job = job_of<SINGLE>(c, jid);
-                const SceneView& s = scene_of<SINGLE>(c, job.scene);
-                const uint32_t local = (uint32_t)(w - job.work_begin);
-                const uint32_t WH = (uint32_t)(job.width * job.height);
-                const uint32_t sample = job.first_sample + local / WH;
-                uint32_t px, py;
-                tile_pixel(local % WH, (uint32_t)job.width, (uint32_t)job.height, &px, &py);
-                const PathState ps = init_path(s, (int)px, (int)py, sample, job.seed);
-                store_new_path(c.P, slot, ps);   // ray, flags, T = 1, L = 0, key
-                // Sample-buffer index (sample, pixel) of this path.
-                c.wk[slot] = make_uint2(jid, (local / WH) * WH + py * (uint32_t)job.width + px);
-                r = ps.ray;
-            } else {
-                slot = A[idx];
-                const float4 o = c.P.ro[slot], d = c.P.rd[slot];
-                r.o = Vec3{o.x, o.y, o.z};
-                r.d = Vec3{d.x, d.y, d.z};
-                r.tmin = kRayEps;
-                r.tmax = kInf;
-                if constexpr (!SINGLE) jid = c.wk[slot].x;
+    const WaveRange wr = wave_range(n_act);
+    if (block_idle(n_act, wr.per)) return;
+    stage_lds<SINGLE>(c, lds, &sp);
+    uint32_t cursor = wr.begin;
+    const uint32_t end = wr.end;
+    LdsNodes src{};
+    int nn = 0, sc = 0;
+    if constexpr (SINGLE) {
+        src = src_of(c.scene0, lds, &sp, 0);
+        nn = c.scene0.n_nodes;
+    }
+    uint32_t item = kIdle, inv = 0;
+    Traversal tr;
+    tr.node = -1;
+    tr.prim = -1;
+    for (;;) {
+        const uint64_t idle = __ballot(item == kIdle);
+        if (idle != 0 && cursor < end) {
+            const uint32_t cand = cursor + mbcnt64(idle);
+            if (item == kIdle && cand < end) {
+                item = cand;
+                Ray r;
+                if (item < n_live) {
+                    const float4 o = in.ro[item], d = in.rd[item];
+                    r.o = Vec3{o.x, o.y, o.z};
+                    r.d = Vec3{d.x, d.y, d.z};
+                    r.tmin = kRayEps;
+                    r.tmax = kInf;
+                    if constexpr (!SINGLE) sc = c.jobs[in.kw[item].z].scene;
+                } else {
+                    uint32_t jid, sidx;
+                    r = new_path<SINGLE>(c, n_jobs, wbase + (item - n_live), &jid, &sidx).ray;
+                    if constexpr (!SINGLE) sc = c.jobs[jid].scene;
+                }
+                if constexpr (!SINGLE) {
+                    src = src_of(c.scenes[sc], lds, &sp, sc);
+                    nn = c.scenes[sc].n_nodes;
+                }
+                tr = traversal_begin(r);
+                if (nn <= 0) tr.node = -1;
             }
-            const int sc = job_of<SINGLE>(c, jid).scene;
-            const SceneView& s = scene_of<SINGLE>(c, sc);
-            Hit h;
-            intersect_with(s, src_of(s, lds, &sp, sc), r, &h, false);
-            if (!hit_is_valid(s, h)) {
+            cursor = min(cursor + (uint32_t)__popcll(idle), end);
+        }
+        if (__ballot(item != kIdle) == 0) break;
+        const int limit = cursor < end ? c.refill : 1;
+        for (;;) {
+            const bool busy = tr.node >= 0;
+            if (__popcll(__ballot(busy)) < limit) break;
+            if (busy) traversal_step(nn, src, tr, false);
+        }
+        if (item != kIdle && tr.node < 0) {
+            Hit h{tr.r.tmax, tr.prim};
+            if (!hit_is_valid(scene_of<SINGLE>(c, sc), h)) {
                 inv += 1;
                 h.prim = -1;
             }
-            c.P.hit[slot] = make_float2(h.t, __int_as_float(h.prim));
-            if (h.prim < 0) {
-                cat = 0;
-            } else {
-                const int mt = s.materials[s.prims[h.prim].material].type;
-                cat = mt == kEmissive ? 0 : mt + 1;
-            }
+            c.hit[item] = make_float2(h.t, __int_as_float(h.prim));
+            item = kIdle;
         }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) wave_push(c.Q[q], &st->q[q], c.cap, slot, valid && cat == q, &st->overflow);
     }
     wave_add_u32(&c.ctr->invalid_hits, inv);
 }
 
+// Classify + shade of items [0, it_act). Survivors are compacted into `out` in item order; slots
+// for survivors and shadow rays are reserved once per block round.
 template <bool SINGLE>
-__global__ __launch_bounds__(512) void pw_shade_kernel(PCtx c, int parity) {
+__global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, PathBuf out) {
+    __shared__ uint32_t cnt[2][3][16];
+    __shared__ uint32_t bases[2][2];
     PState* st = c.st;
-    const uint32_t n0 = st->q[0], n1 = st->q[1], n2 = st->q[2], n3 = st->q[3];
-    const uint32_t e1 = n0 + n1, e2 = e1 + n2, total = e2 + n3;
-    uint32_t* An = parity ? c.A[0] : c.A[1];
-    uint32_t* Ffree = c.F + st->it_free_after;
+    const uint32_t n = st->it_act, n_live = st->it_live, n_jobs = st->n_jobs;
+    const unsigned long long wbase = st->it_work_base;
+    const uint32_t wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < total; b0 += stride) {
-        const uint32_t idx = b0 + threadIdx.x;
-        const bool valid = idx < total;
-        uint32_t slot = 0, jid = 0;
+    int round = 0;
+    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n; b0 += stride, round ^= 1) {
+        const uint32_t item = b0 + threadIdx.x;
         bool live = false, fin = false;
-        int has = 0, sc = 0;
-        Ray sh;
-        Vec3 contrib;
-        if (valid) {
-            const uint32_t* Qp;
-            uint32_t qi;
-            if (idx < n0) { Qp = c.Q[0]; qi = idx; }
-            else if (idx < e1) { Qp = c.Q[1]; qi = idx - n0; }
-            else if (idx < e2) { Qp = c.Q[2]; qi = idx - e1; }
-            else { Qp = c.Q[3]; qi = idx - e2; }
-            slot = Qp[qi];
-            const uint2 w = c.wk[slot];
-            if constexpr (!SINGLE) jid = w.x;
+        int has = 0;
+        uint32_t jid = 0, sidx = 0;
+        PathState ps{};
+        PathKey key{0u, 0u};
+        Ray sh{};
+        Vec3 contrib{0.0f, 0.0f, 0.0f};
+        if (item < n) {
+            const float2 hv = c.hit[item];
+            if (item < n_live) {
+                const float4 o = in.ro[item], d = in.rd[item], t = in.T[item], l = in.L[item];
+                const uint4 kw = in.kw[item];
+                ps.ray.o = Vec3{o.x, o.y, o.z};
+                ps.ray.d = Vec3{d.x, d.y, d.z};
+                ps.ray.tmin = kRayEps;
+                ps.ray.tmax = kInf;
+                ps.prev_pdf = o.w;
+                const int bits = __float_as_int(d.w);
+                ps.depth = bits & 0xffff;
+                ps.specular = bits >> 16;
+                ps.throughput = Vec3{t.x, t.y, t.z};
+                ps.L = Vec3{l.x, l.y, l.z};
+                ps.key = PathKey{kw.x, kw.y};
+                jid = kw.z;
+                sidx = kw.w;
+            } else {
+                ps = new_path<SINGLE>(c, n_jobs, wbase + (item - n_live), &jid, &sidx);
+            }
+            key = ps.key;
             const JobDev& job = job_of<SINGLE>(c, jid);
-            sc = job.scene;
-            const SceneView& s = scene_of<SINGLE>(c, sc);
-            PathState ps = load_path(c.P, slot, true);
-            const float2 hv = c.P.hit[slot];
+            const SceneView& s = scene_of<SINGLE>(c, job.scene);
             const Hit h{hv.x, __float_as_int(hv.y)};
             Surface f;
             const int mat = classify(s, ps, h, job.max_depth, &f);
             if (mat == kLambert) live = shade_lambert(s, ps, f, &sh, &contrib, &has);
             else if (mat == kMetal) live = shade_metal(s, ps, f);
             else if (mat == kDielectric) live = shade_dielectric(s, ps, f);
-            if (live) store_scatter(c.P, slot, ps);
-            const float4 L = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
             fin = !live && !has;
-            if (fin) job.sbuf[w.y] = L;
-            else c.P.L[slot] = L;
+            if (!live) job.sbuf[sidx] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
         }
-        wave_push(An, &st->n_next, c.cap, slot, live, &st->overflow);
-        wave_push(Ffree, &st->n_freed, c.cap, slot, fin, &st->overflow);
-        count_done(c.job_done, fin, jid);
-        const uint32_t e = wave_reserve(&st->q[4], has != 0);
+        const uint64_t ml = __ballot(live), ms = __ballot(has != 0), mf = __ballot(fin);
+        uint32_t(*cn)[16] = cnt[round];
+        if (lane_id() == 0) {
+            cn[0][wave] = (uint32_t)__popcll(ml);
+            cn[1][wave] = (uint32_t)__popcll(ms);
+            cn[2][wave] = (uint32_t)__popcll(mf);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t0 = 0, t1 = 0, t2 = 0;
+            for (uint32_t w = 0; w < nwaves; ++w) {
+                const uint32_t a = cn[0][w], b = cn[1][w], d = cn[2][w];
+                cn[0][w] = t0;
+                cn[1][w] = t1;
+                t0 += a;
+                t1 += b;
+                t2 += d;
+            }
+            bases[round][0] = t0 ? atomicAdd(&st->n_next, t0) : 0u;
+            bases[round][1] = t1 ? atomicAdd(&st->n_shadow, t1) : 0u;
+            if constexpr (SINGLE)
+                if (t2) atomicAdd(&c.job_done[0], t2);
+        }
+        __syncthreads();
+        uint32_t o = 0;
+        if (live) {
+            o = bases[round][0] + cn[0][wave] + mbcnt64(ml);
+            out.ro[o] = make_float4(ps.ray.o.x, ps.ray.o.y, ps.ray.o.z, ps.prev_pdf);
+            out.rd[o] = make_float4(ps.ray.d.x, ps.ray.d.y, ps.ray.d.z,
+                                    __int_as_float(ps.depth | (ps.specular << 16)));
+            out.T[o] = make_float4(ps.throughput.x, ps.throughput.y, ps.throughput.z, 0.0f);
+            out.L[o] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
+            out.kw[o] = make_uint4(key.k0, key.k1, jid, sidx);
+        }
         if (has) {
+            const uint32_t e = bases[round][1] + cn[1][wave] + mbcnt64(ms);
             c.SQ.o[e] = make_float4(sh.o.x, sh.o.y, sh.o.z, sh.tmax);
-            c.SQ.d[e] = make_float4(sh.d.x, sh.d.y, sh.d.z, __int_as_float((int)(slot | (live ? 0u : 0x80000000u))));
-            c.SQ.c[e] = make_float4(contrib.x, contrib.y, contrib.z, __int_as_float(sc));
+            c.SQ.d[e] = make_float4(sh.d.x, sh.d.y, sh.d.z, __int_as_float((int)(jid | (live ? 0u : kFinalFlag))));
+            c.SQ.c[e] = make_float4(contrib.x, contrib.y, contrib.z, __int_as_float((int)(live ? o : sidx)));
         }
+        if constexpr (!SINGLE) count_done(c.job_done, fin, jid);
     }
 }
 
+// Any-hit traversal of the shadow rays, "while-while" like extend. An unoccluded ray adds its
+// contribution to the survivor's radiance in `out`, or to the sample of a path that ended with it.
 template <bool SINGLE>
-__global__ __launch_bounds__(512) void pw_shadow_kernel(PCtx c) {
+__global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
     extern __shared__ Node lds[];
     __shared__ LdsPlan sp;
+    const uint32_t n = c.st->n_shadow;
+    const WaveRange wr = wave_range(n);
+    if (block_idle(n, wr.per)) return;
     stage_lds<SINGLE>(c, lds, &sp);
-    PState* st = c.st;
-    const uint32_t n = st->q[4];
-    uint32_t* Ffree = c.F + st->it_free_after;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n; b0 += stride) {
-        const uint32_t idx = b0 + threadIdx.x;
-        uint32_t slot = 0, jid = 0;
-        bool fin = false;
-        if (idx < n) {
-            const float4 o = c.SQ.o[idx], d = c.SQ.d[idx], cc = c.SQ.c[idx];
-            const uint32_t tag = (uint32_t)__float_as_int(d.w);
-            slot = tag & 0x7fffffffu;
-            fin = (tag >> 31) != 0;
-            const int sc = SINGLE ? 0 : __float_as_int(cc.w);
-            const SceneView& s = scene_of<SINGLE>(c, sc);
-            Ray r;
-            r.o = Vec3{o.x, o.y, o.z};
-            r.d = Vec3{d.x, d.y, d.z};
-            r.tmin = kRayEps;
-            r.tmax = o.w;
-            Hit h;
-            const bool blocked = intersect_with(s, src_of(s, lds, &sp, sc), r, &h, true);
-            if (!blocked || fin) {
-                float4 l = c.P.L[slot];
-                if (!blocked) {
-                    l.x += cc.x;
-                    l.y += cc.y;
-                    l.z += cc.z;
-                }
-                if (fin) {
-                    const uint2 w = c.wk[slot];
-                    if constexpr (!SINGLE) jid = w.x;
-                    job_of<SINGLE>(c, jid).sbuf[w.y] = l;
-                } else {
-                    c.P.L[slot] = l;
-                }
-            }
-        }
-        wave_push(Ffree, &st->n_freed, c.cap, slot, fin, &st->overflow);
-        count_done(c.job_done, fin, jid);
+    uint32_t cursor = wr.begin;
+    const uint32_t end = wr.end;
+    LdsNodes src{};
+    int nn = 0;
+    if constexpr (SINGLE) {
+        src = src_of(c.scene0, lds, &sp, 0);
+        nn = c.scene0.n_nodes;
     }
+    uint32_t item = kIdle, tag = 0, ndone = 0;
+    Traversal tr;
+    tr.node = -1;
+    tr.prim = -1;
+    for (;;) {
+        const uint64_t idle = __ballot(item == kIdle);
+        if (idle != 0 && cursor < end) {
+            const uint32_t cand = cursor + mbcnt64(idle);
+            if (item == kIdle && cand < end) {
+                item = cand;
+                const float4 o = c.SQ.o[item], d = c.SQ.d[item];
+                tag = (uint32_t)__float_as_int(d.w);
+                Ray r;
+                r.o = Vec3{o.x, o.y, o.z};
+                r.d = Vec3{d.x, d.y, d.z};
+                r.tmin = kRayEps;
+                r.tmax = o.w;
+                if constexpr (!SINGLE) {
+                    const int sc = c.jobs[tag & ~kFinalFlag].scene;
+                    src = src_of(c.scenes[sc], lds, &sp, sc);
+                    nn = c.scenes[sc].n_nodes;
+                }
+                tr = traversal_begin(r);
+                if (nn <= 0) tr.node = -1;
+            }
+            cursor = min(cursor + (uint32_t)__popcll(idle), end);
+        }
+        if (__ballot(item != kIdle) == 0) break;
+        const int limit = cursor < end ? c.refill : 1;
+        for (;;) {
+            const bool busy = tr.node >= 0;
+            if (__popcll(__ballot(busy)) < limit) break;
+            if (busy) traversal_step(nn, src, tr, true);
+        }
+        bool fin = false;
+        uint32_t jid = 0;
+        if (item != kIdle && tr.node < 0) {
+            fin = (tag & kFinalFlag) != 0;
+            if constexpr (!SINGLE) jid = tag & ~kFinalFlag;
+            if (tr.prim < 0) {
+                const float4 cc = c.SQ.c[item];
+                const uint32_t dest = (uint32_t)__float_as_int(cc.w);
+                float4* dst = fin ? &job_of<SINGLE>(c, jid).sbuf[dest] : &out.L[dest];
+                float4 l = *dst;
+                l.x += cc.x;
+                l.y += cc.y;
+                l.z += cc.z;
+                *dst = l;
+            }
+            item = kIdle;
+        }
+        if constexpr (SINGLE) ndone += fin ? 1u : 0u;
+        else count_done(c.job_done, fin, jid);
+    }
+    if constexpr (SINGLE) wave_add_u32(&c.job_done[0], ndone);
 }
 
 // Adds a finished job's samples to its film, per pixel in sample order.
@@ -361,11 +463,14 @@ struct PersistentEngine::Impl {
     hipStream_t stream = nullptr;
     DevCounters* ctr = nullptr;
     int block = kDefaultBlock, graph_iters = kDefaultGraphIters, lds_bytes = kDefaultLdsKiB * 1024, cus = 0;
+    int refill = kDefaultRefill;
     std::vector<void*> pool_allocs;
     PState* st = nullptr;
     PState* st_host = nullptr;       // pinned mirror
     uint32_t* done_host = nullptr;   // pinned mirror of job_done
     PCtx ctx{};
+    PathBuf S[2];
+    int par = 0;   // buffer holding the survivors of the last iteration
     LdsPlan lds_plan{};
     struct HostJob {
         JobDev dev;
@@ -426,12 +531,19 @@ void PersistentEngine::ensure_pool(const PersistentTuning& t) {
     int cap = t.pool_paths > 0 ? t.pool_paths : kDefaultPool;
     cap = (cap + 1023) & ~1023;
     m.block = t.block_size > 0 ? t.block_size : kDefaultBlock;
-    if (m.block != 128 && m.block != 256 && m.block != 512) throw std::runtime_error("persistent: block size must be 128, 256 or 512");
+    if (m.block != 256 && m.block != 512 && m.block != 1024)
+        throw std::runtime_error("persistent: block size must be 256, 512 or 1024");
+    m.refill = t.refill > 0 ? t.refill : kDefaultRefill;
+    if (m.refill > 64) throw std::runtime_error("persistent: refill threshold must be 1..64 lanes");
+    m.ctx.refill = m.refill;
     m.graph_iters = t.graph_iters > 0 ? ((t.graph_iters + 1) & ~1) : kDefaultGraphIters;
     const int kib = t.lds_kib > 0 ? t.lds_kib : kDefaultLdsKiB;
     if (kib > 64) throw std::runtime_error("persistent: at most 64 KiB of LDS per workgroup");
     m.lds_bytes = kib * 1024;
-    if (cap == cap_ && !m.pool_allocs.empty()) return;
+    if (cap == cap_ && !m.pool_allocs.empty()) {
+        m.free_graph();   // the captured graph bakes in the launch shape and refill threshold
+        return;
+    }
     m.free_graph();
     m.free_pool();
     pool_bytes_ = 0;
@@ -446,20 +558,18 @@ void PersistentEngine::ensure_pool(const PersistentTuning& t) {
     const size_t P = (size_t)cap;
     PCtx& c = m.ctx;
     c = PCtx{};
-    alloc(&c.P.ro, P * 16);
-    alloc(&c.P.rd, P * 16);
-    alloc(&c.P.T, P * 16);
-    alloc(&c.P.L, P * 16);
-    alloc(&c.P.key, P * 8);
-    alloc(&c.P.hit, P * 8);
+    c.refill = m.refill;
+    for (PathBuf& b : m.S) {
+        alloc(&b.ro, P * 16);
+        alloc(&b.rd, P * 16);
+        alloc(&b.T, P * 16);
+        alloc(&b.L, P * 16);
+        alloc(&b.kw, P * 16);
+    }
+    alloc(&c.hit, P * 8);
     alloc(&c.SQ.o, P * 16);
     alloc(&c.SQ.d, P * 16);
     alloc(&c.SQ.c, P * 16);
-    alloc(&c.wk, P * 8);
-    alloc(&c.A[0], P * 4);
-    alloc(&c.A[1], P * 4);
-    alloc(&c.F, P * 4);
-    for (int q = 0; q < 4; ++q) alloc(&c.Q[q], P * 4);
     alloc(&c.st, sizeof(PState));
     alloc(&c.jobs, sizeof(JobDev) * kMaxJobs);
     alloc(&c.job_done, sizeof(uint32_t) * kMaxJobs);
@@ -491,7 +601,8 @@ void PersistentEngine::reset_jobs() {
     m.sbuf_bytes = 0;
     m.total_work = 0;
     m.free_graph();
-    pw_init_kernel<<<(cap_ + 255) / 256, 256, 0, m.stream>>>(m.st, m.ctx.F, (uint32_t)cap_);
+    pw_init_kernel<<<1, 1, 0, m.stream>>>(m.st);
+    m.par = 0;
     check(hipGetLastError(), "persistent init");
     update_lds_plan();
 }
@@ -588,23 +699,26 @@ PersistentEngine::LaunchPlan PersistentEngine::make_plan(bool single) {
     return L;
 }
 
-// One iteration; with `ev` (5 events) the stage boundaries are recorded for profiling:
+// One iteration reading survivors from buffer `par` and writing them to the other; with `ev`
+// (5 events) the stage boundaries are recorded for profiling:
 // ev[0] plan ev[1] extend ev[2] shade ev[3] shadow ev[4].
 void PersistentEngine::launch_iteration(const LaunchPlan& L, int par, hipEvent_t* ev) {
     Impl& m = *m_;
     hipStream_t st = m.stream;
     const PCtx& c = L.c;
+    const PathBuf& in = m.S[par & 1];
+    const PathBuf& out = m.S[(par & 1) ^ 1];
     if (ev) check(hipEventRecord(ev[0], st), "hipEventRecord");
-    pw_plan_kernel<<<1, 1, 0, st>>>(m.st, m.ctr);
+    pw_plan_kernel<<<1, 1, 0, st>>>(m.st, m.ctr, c.cap);
     if (ev) check(hipEventRecord(ev[1], st), "hipEventRecord");
-    if (L.single) pw_extend_kernel<true><<<L.gx, m.block, L.lds_bytes, st>>>(c, par);
-    else pw_extend_kernel<false><<<L.gx, m.block, L.lds_bytes, st>>>(c, par);
+    if (L.single) pw_extend_kernel<true><<<L.gx, m.block, L.lds_bytes, st>>>(c, in);
+    else pw_extend_kernel<false><<<L.gx, m.block, L.lds_bytes, st>>>(c, in);
     if (ev) check(hipEventRecord(ev[2], st), "hipEventRecord");
-    if (L.single) pw_shade_kernel<true><<<L.gs, m.block, 0, st>>>(c, par);
-    else pw_shade_kernel<false><<<L.gs, m.block, 0, st>>>(c, par);
+    if (L.single) pw_shade_kernel<true><<<L.gs, m.block, 0, st>>>(c, in, out);
+    else pw_shade_kernel<false><<<L.gs, m.block, 0, st>>>(c, in, out);
     if (ev) check(hipEventRecord(ev[3], st), "hipEventRecord");
-    if (L.single) pw_shadow_kernel<true><<<L.gw, m.block, L.lds_bytes, st>>>(c);
-    else pw_shadow_kernel<false><<<L.gw, m.block, L.lds_bytes, st>>>(c);
+    if (L.single) pw_shadow_kernel<true><<<L.gw, m.block, L.lds_bytes, st>>>(c, out);
+    else pw_shadow_kernel<false><<<L.gw, m.block, L.lds_bytes, st>>>(c, out);
     if (ev) check(hipEventRecord(ev[4], st), "hipEventRecord");
 }
 
@@ -638,8 +752,9 @@ void PersistentEngine::run_profiled(StageTimes* out) {
     const LaunchPlan L = make_plan(single);
     hipEvent_t ev[5];
     for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
-    for (int it = 0;; ++it) {
-        launch_iteration(L, it & 1, ev);
+    for (;;) {
+        launch_iteration(L, m.par, ev);
+        m.par ^= 1;
         check(hipMemcpyAsync(m.st_host, m.st, sizeof(PState), hipMemcpyDeviceToHost, m.stream), "copy state");
         check(hipStreamSynchronize(m.stream), "profile sync");
         const PState& s = *m.st_host;
@@ -651,16 +766,30 @@ void PersistentEngine::run_profiled(StageTimes* out) {
         out->shade_lambert += ms[2];
         out->shadow += ms[3];
         out->items[0] += s.it_act;
-        out->items[2] += (long long)s.q[0] + s.q[1] + s.q[2] + s.q[3];
-        out->items[5] += s.q[4];
+        out->items[2] += s.it_act;
+        out->items[5] += s.n_shadow;
+        // Algorithmic bytes (PathBuf and ShadowQueue layouts; BVH and primitives come from LDS):
+        //   extend  survivor ray 32 (new paths are generated), hit 8
+        //   shade   hit 8; survivor state 80 in, 80 out; sample 16 per finished path; shadow ray 48
+        //   shadow  ray 32 (origin, direction); the contribution and the radiance read-modify-write
+        //           of unoccluded rays are not counted
+        const long long live = s.it_live, act = s.it_act, next = s.n_next, sh = s.n_shadow;
+        out->bytes[0] += 32 * live + 8 * act;
+        out->bytes[2] += 8 * act + 80 * live + 80 * next + 16 * (act - next) + 48 * sh;
+        out->bytes[5] += 32 * sh;
         out->items[7] += 1;
         out->launches[0] += 1;
         out->launches[2] += 1;
         out->launches[5] += 1;
         out->launches[7] += 1;
     }
+    // The captured graph runs an even number of iterations from buffer 0; the work has drained,
+    // so one more (empty) iteration restores that parity.
+    if (m.par) {
+        launch_iteration(L, m.par, nullptr);
+        m.par = 0;
+    }
     for (auto& e : ev) (void)hipEventDestroy(e);
-    // Fold the last iteration's counts (the plan of the final, empty iteration did so on device).
 }
 
 PersistentProgress PersistentEngine::progress() {
@@ -766,39 +895,54 @@ PersistentTuning autotune_persistent(const PersistentJob* jobs, int n_jobs, hipS
         j.spp = std::min(j.spp, probe_spp);
         work += (uint64_t)j.scene->width * j.scene->height * j.spp;
     }
-    PersistentTuning best;
-    double best_ms = 1e30;
-    const int blocks[] = {256, 512};
-    const int pools[] = {1 << 20, 1 << 21, 1 << 22};
-    const int ldss[] = {32, 64};
     PersistentEngine eng;
     hipEvent_t a, b;
     check(hipEventCreate(&a), "hipEventCreate");
     check(hipEventCreate(&b), "hipEventCreate");
+    PersistentTuning best;
+    double best_ms = 1e30;
     // Probe renders add into the jobs' films; the caller resets them afterwards.
-    for (int bs : blocks)
-        for (int pool : pools) {
-            if ((uint64_t)pool > 2 * work && pool != pools[0]) continue;
-            for (int kib : ldss) {
-                PersistentTuning t;
-                t.block_size = bs;
-                t.pool_paths = pool;
-                t.lds_kib = kib;
-                PersistentResult r;
-                eng.render(probe.data(), n_jobs, t, ctr, stream, &r);   // warm-up (graph capture)
-                check(hipEventRecord(a, stream), "hipEventRecord");
-                eng.render(probe.data(), n_jobs, t, ctr, stream, &r);
-                check(hipEventRecord(b, stream), "hipEventRecord");
-                check(hipEventSynchronize(b), "hipEventSynchronize");
-                float ms = 0.0f;
-                check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
-                if (trials) trials->push_back({t, (double)ms});
-                if (ms < best_ms) {
-                    best_ms = ms;
-                    best = t;
-                }
-            }
+    auto time = [&](const PersistentTuning& t) {
+        PersistentResult r;
+        eng.render(probe.data(), n_jobs, t, ctr, stream, &r);   // warm-up (graph capture)
+        check(hipEventRecord(a, stream), "hipEventRecord");
+        eng.render(probe.data(), n_jobs, t, ctr, stream, &r);
+        check(hipEventRecord(b, stream), "hipEventRecord");
+        check(hipEventSynchronize(b), "hipEventSynchronize");
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+        if (trials) trials->push_back({t, (double)ms});
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = t;
         }
+    };
+    // Stage 1: workgroup size x refill threshold at the largest pool and LDS budget.
+    const int pools[] = {1 << 20, 1 << 21, 1 << 22};
+    int top = pools[0];
+    for (int p : pools)
+        if ((uint64_t)p <= 2 * work) top = p;
+    for (int bs : {256, 512, 1024})
+        for (int rf : {16, 32, 48}) {
+            PersistentTuning t;
+            t.block_size = bs;
+            t.refill = rf;
+            t.pool_paths = top;
+            t.lds_kib = 64;
+            time(t);
+        }
+    // Stage 2: pool size (queue capacity) and LDS budget for the best shape.
+    const PersistentTuning shape = best;
+    for (int p : pools) {
+        if (p > top) continue;
+        for (int kib : {32, 64}) {
+            PersistentTuning t = shape;
+            t.pool_paths = p;
+            t.lds_kib = kib;
+            if (p == shape.pool_paths && kib == shape.lds_kib) continue;
+            time(t);
+        }
+    }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     (void)hipFree(ctr);

@@ -25,6 +25,7 @@ struct RenderParams {
     int pool_paths = 0;
     int scan_tile = 0;           // plain wavefront compaction tile
     int lds_kib = 0;             // persistent: LDS per workgroup (KiB)
+    int refill = 0;              // persistent: traversal refill threshold (lanes)
     int lds_nodes = -1;          // nodes staged in LDS per workgroup (-1 = default)
 };
 
@@ -34,6 +35,8 @@ struct StageTimes {   // milliseconds summed over a render
     // Items processed and kernel launches per stage, same order as the fields above.
     long long items[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     long long launches[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // Algorithmic device-memory bytes per stage (persistent engine; 0 where not computed).
+    long long bytes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 struct RenderStats {

@@ -145,6 +145,7 @@ void GpuRenderer::run_passes(int first_sample, int count, const RenderParams& p,
             tune.block_size = p.block_size;
             tune.pool_paths = p.pool_paths;
             tune.lds_kib = p.lds_kib;
+            tune.refill = p.refill;
             gpu::PersistentResult pr;
             m.pe->render(&job, 1, tune, m.d_ctr, m.stream, &pr, p.stage_timing ? &st : nullptr);
             iterations = pr.iterations;
@@ -177,6 +178,7 @@ void GpuRenderer::run_passes(int first_sample, int count, const RenderParams& p,
         for (int k = 0; k < 8; ++k) {
             stats->stages.items[k] += st.items[k];
             stats->stages.launches[k] += st.launches[k];
+            stats->stages.bytes[k] += st.bytes[k];
         }
     }
 }

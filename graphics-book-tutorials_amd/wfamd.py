@@ -33,25 +33,27 @@ class RenderParams(C.Structure):
     _fields_ = [("spp", C.c_int), ("seed", C.c_uint32), ("max_depth", C.c_int), ("threads", C.c_int),
                 ("count_rays", C.c_int), ("stage_timing", C.c_int), ("queue_capacity", C.c_int),
                 ("trace_bounces", C.c_int), ("block_size", C.c_int), ("pool_paths", C.c_int),
-                ("scan_tile", C.c_int), ("lds_nodes", C.c_int), ("lds_kib", C.c_int)]
+                ("scan_tile", C.c_int), ("lds_nodes", C.c_int), ("lds_kib", C.c_int), ("refill", C.c_int)]
 
 
 class RenderStats(C.Structure):
     _fields_ = [("render_ms", C.c_double), ("total_ms", C.c_double), ("extension_rays", C.c_uint64),
                 ("shadow_rays", C.c_uint64), ("queue_high_water", C.c_int64 * 5), ("queue_overflows", C.c_int64),
                 ("invalid_hits", C.c_int64), ("peak_vram_bytes", C.c_uint64), ("stage_ms", C.c_double * 8),
-                ("iterations", C.c_int64), ("stage_items", C.c_int64 * 8), ("stage_launches", C.c_int64 * 8)]
+                ("iterations", C.c_int64), ("stage_items", C.c_int64 * 8), ("stage_launches", C.c_int64 * 8),
+                ("stage_bytes", C.c_int64 * 8)]
 
     STAGES = ("extend", "classify", "shade_lambert", "shade_metal", "shade_dielectric", "shadow", "compact", "other")
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
-        for k in ("queue_high_water", "stage_ms", "stage_items", "stage_launches"):
+        for k in ("queue_high_water", "stage_ms", "stage_items", "stage_launches", "stage_bytes"):
             d[k] = list(getattr(self, k))
         return d
 
     def stages(self):
-        return {n: {"ms": self.stage_ms[i], "items": self.stage_items[i], "launches": self.stage_launches[i]}
+        return {n: {"ms": self.stage_ms[i], "items": self.stage_items[i], "launches": self.stage_launches[i],
+                    "bytes": self.stage_bytes[i]}
                 for i, n in enumerate(self.STAGES) if self.stage_launches[i] > 0}
 
 
@@ -90,7 +92,7 @@ class BatchStats(C.Structure):
     _fields_ = [("render_ms", C.c_double), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("invalid_hits", C.c_int64), ("queue_overflows", C.c_int64), ("iterations", C.c_int64),
                 ("graph_launches", C.c_int64), ("block_size", C.c_int), ("pool_paths", C.c_int),
-                ("lds_kib", C.c_int), ("device_bytes", C.c_uint64)]
+                ("lds_kib", C.c_int), ("refill", C.c_int), ("device_bytes", C.c_uint64)]
 
 
 class ServerStats(C.Structure):
@@ -189,9 +191,9 @@ def _iptr(a):
 
 
 def params(spp=64, seed=1234, max_depth=8, threads=0, stage_timing=False, queue_capacity=0, trace_bounces=False,
-           block_size=0, pool_paths=0, scan_tile=0, lds_nodes=-1, lds_kib=0):
+           block_size=0, pool_paths=0, scan_tile=0, lds_nodes=-1, lds_kib=0, refill=0):
     return RenderParams(spp, seed, max_depth, threads, 0, int(stage_timing), queue_capacity, int(trace_bounces),
-                        block_size, pool_paths, scan_tile, lds_nodes, lds_kib)
+                        block_size, pool_paths, scan_tile, lds_nodes, lds_kib, refill)
 
 
 class Scene:

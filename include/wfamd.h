@@ -47,11 +47,12 @@ typedef struct wf_render_params {
     int stage_timing;     /* wavefront: per-stage event timing (synchronising) */
     int queue_capacity;   /* wavefront test hook: 0 = path count; smaller -> counted overflows */
     int trace_bounces;    /* wavefront: record per-bounce integer work, see wf_renderer_trace */
-    int block_size;       /* persistent: threads per workgroup (128, 256, 512) */
+    int block_size;       /* persistent: threads per workgroup (256, 512, 1024) */
     int pool_paths;       /* persistent: path slots (= queue capacity) */
     int scan_tile;        /* plain wavefront: compaction tile (elements) */
     int lds_nodes;        /* megakernel / plain wavefront: BVH nodes staged in LDS, -1 = default */
     int lds_kib;          /* persistent: LDS per workgroup for BVH nodes + primitive records (KiB) */
+    int refill;           /* persistent: a wave refetches rays when fewer lanes than this traverse */
 } wf_render_params;
 
 /* RenderStats, include/wf/renderer.hpp:23-37. stage_ms: extend, classify, shade_lambert,
@@ -72,6 +73,7 @@ typedef struct wf_render_stats {
     int64_t iterations;
     int64_t stage_items[8];
     int64_t stage_launches[8];
+    int64_t stage_bytes[8];      /* persistent engine: algorithmic device-memory bytes per stage */
 } wf_render_stats;
 
 typedef struct wf_scene_info {
@@ -203,7 +205,7 @@ typedef struct wf_batch_stats {
     uint64_t extension_rays, shadow_rays;
     int64_t invalid_hits, queue_overflows;
     int64_t iterations, graph_launches;
-    int block_size, pool_paths, lds_kib;
+    int block_size, pool_paths, lds_kib, refill;
     uint64_t device_bytes;
 } wf_batch_stats;
 
