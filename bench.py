@@ -71,6 +71,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-spp", type=int, default=16, help="samples per pixel of the CPU baseline sample")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run the plain-copy calibration kernel over BYTES (traffic-counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (the path shards by pixel tiles; see DESIGN.md); "
@@ -80,6 +82,11 @@ def main():
     import wfamd
 
     wfamd.lib()
+    if args.copy_probe:
+        ms = wfamd.copy_probe(args.copy_probe, reps=args.steps)
+        print(json.dumps({"copy_probe_bytes": args.copy_probe, "reps": args.steps, "ms_per_copy": ms,
+                          "gbs": 2 * args.copy_probe / (ms / 1e3) / 1e9}), flush=True)
+        return
     dev = wfamd.query_device()
     print(f"[bench] device {dev['name']} {dev['arch']} {dev['compute_units']} CUs", file=sys.stderr, flush=True)
     scene = wfamd.Scene(args.scene, args.width, args.height)
@@ -88,11 +95,11 @@ def main():
     tuning = wfamd.params(spp=spp, seed=seed, max_depth=depth)
     if not args.no_autotune:
         t0 = time.perf_counter()
-        tuned = wfamd.autotune([(scene, spp, seed, depth)], probe_spp=4)
+        tuned = wfamd.autotune([(scene, spp, seed, depth)], probe_spp=16)
         tuning.block_size, tuning.pool_paths, tuning.lds_kib = tuned.block_size, tuned.pool_paths, tuned.lds_kib
-        tuning.refill = tuned.refill
+        tuning.refill, tuning.leaf_batch, tuning.dir_bins = tuned.refill, tuned.leaf_batch, tuned.dir_bins
         print(f"[bench] autotune: block {tuned.block_size} pool {tuned.pool_paths} lds {tuned.lds_kib} KiB "
-              f"refill {tuned.refill} "
+              f"refill {tuned.refill} leaf batch {tuned.leaf_batch} dir bins {tuned.dir_bins} "
               f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
 
     r = wfamd.Renderer(scene, "persistent")
@@ -136,7 +143,8 @@ def main():
     # (computed by the engine from each iteration's counts; layouts in wf_persistent.hip).
     roof = None
     prof = wfamd.params(spp=spp, seed=seed, max_depth=depth, stage_timing=True, block_size=tuning.block_size,
-                        pool_paths=tuning.pool_paths, lds_kib=tuning.lds_kib, refill=tuning.refill)
+                        pool_paths=tuning.pool_paths, lds_kib=tuning.lds_kib, refill=tuning.refill,
+                        leaf_batch=tuning.leaf_batch, dir_bins=tuning.dir_bins)
     r.reset()
     pst = r.run_passes(0, spp, prof)
     names = {"extend": "extend", "shade_lambert": "shade", "shadow": "shadow"}
@@ -167,7 +175,7 @@ def main():
         "config": {"workload": f"{args.scene} {args.width}x{args.height} {spp}spp depth {depth} seed {seed} "
                                f"(material-divergent scene)", "arm": "persistent batched wavefront",
                    "block_size": tuning.block_size, "pool_paths": tuning.pool_paths, "lds_kib": tuning.lds_kib,
-                   "refill": tuning.refill,
+                   "refill": tuning.refill, "leaf_batch": tuning.leaf_batch, "dir_bins": tuning.dir_bins,
                    "parallelism": "single GPU"},
         "p50_ms": percentile(lat, 0.5), "p99_ms": percentile(lat, 0.99),
         "extension_mrays_per_s": ext / elapsed / 1e6, "rays_per_render": rays_per_render,

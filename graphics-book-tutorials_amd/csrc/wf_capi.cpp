@@ -64,6 +64,8 @@ wf::RenderParams to_params(const wf_render_params* p) {
     r.lds_nodes = p->lds_nodes;
     r.lds_kib = p->lds_kib;
     r.refill = p->refill;
+    r.leaf_batch = p->leaf_batch;
+    r.dir_bins = p->dir_bins;
     return r;
 }
 
@@ -608,6 +610,8 @@ wf::gpu::PersistentTuning to_tuning(const wf_render_params* p) {
         t.pool_paths = p->pool_paths;
         t.lds_kib = p->lds_kib;
         t.refill = p->refill;
+        t.leaf_batch = p->leaf_batch;
+        t.dir_bins = p->dir_bins;
     }
     return t;
 }
@@ -683,6 +687,8 @@ extern "C" int wf_batch_render(const wf_job* jobs, int n_jobs, const wf_render_p
             stats->pool_paths = eng.pool_paths();
             stats->lds_kib = t.lds_kib;
             stats->refill = t.refill;
+            stats->leaf_batch = t.leaf_batch;
+            stats->dir_bins = t.dir_bins;
             stats->device_bytes = eng.bytes() + b.scenes.bytes();
         }
         return (int)WF_OK;
@@ -699,6 +705,8 @@ extern "C" int wf_autotune(const wf_job* jobs, int n_jobs, int probe_spp, wf_ren
         tuning->pool_paths = best.pool_paths;
         tuning->lds_kib = best.lds_kib;
         tuning->refill = best.refill;
+        tuning->leaf_batch = best.leaf_batch;
+        tuning->dir_bins = best.dir_bins;
         return (int)WF_OK;
     });
 }

@@ -299,28 +299,26 @@ WF_HD bool hit_triangle(Vec3 p0, Vec3 p1, Vec3 p2, const Ray& r, float* t_out) {
 }
 
 // Slab test against [tmin, tmax].
-// The reference returns early once t0 > t1 after an axis. This form evaluates all three axes and
-// compares once (no divergent branches on a GPU); the result is the same: after t0 > t1 the
-// remaining axes can only raise t0 or lower t1 (a NaN slab, 0 * inf, fails both comparisons and
-// changes neither), so the final t0 <= t1 is false exactly when an early exit would have been taken.
+// The reference returns early once t0 > t1 after an axis and updates t0 = tn > t0 ? tn : t0,
+// t1 = tf < t1 ? tf : t1. This form evaluates all three axes and compares once (no divergent
+// branches on a GPU) and folds the updates into fmaxf/fminf; the result is the same:
+//  - after t0 > t1 the remaining axes can only raise t0 or lower t1, so the final t0 <= t1 is
+//    false exactly when an early exit would have been taken;
+//  - t0 and t1 start finite and never become NaN, and a NaN slab bound (0 * inf) leaves them
+//    unchanged both in the select form (its comparison is false) and in fmaxf/fminf (which return
+//    the other operand); the two forms can differ only in the sign of a zero, which no comparison
+//    sees. The swap of tn and tf stays a select: a NaN there must not move the other bound.
 WF_HD bool hit_box(Vec3 lo, Vec3 hi, Vec3 o, Vec3 inv_d, float tmin, float tmax) {
-    float t0 = tmin, t1 = tmax;
     float tn = (lo.x - o.x) * inv_d.x, tf = (hi.x - o.x) * inv_d.x;
-    float a = tn > tf ? tf : tn, b = tn > tf ? tn : tf;
-    t0 = a > t0 ? a : t0;
-    t1 = b < t1 ? b : t1;
+    const float ax = tn > tf ? tf : tn, bx = tn > tf ? tn : tf;
     tn = (lo.y - o.y) * inv_d.y;
     tf = (hi.y - o.y) * inv_d.y;
-    a = tn > tf ? tf : tn;
-    b = tn > tf ? tn : tf;
-    t0 = a > t0 ? a : t0;
-    t1 = b < t1 ? b : t1;
+    const float ay = tn > tf ? tf : tn, by = tn > tf ? tn : tf;
     tn = (lo.z - o.z) * inv_d.z;
     tf = (hi.z - o.z) * inv_d.z;
-    a = tn > tf ? tf : tn;
-    b = tn > tf ? tn : tf;
-    t0 = a > t0 ? a : t0;
-    t1 = b < t1 ? b : t1;
+    const float az = tn > tf ? tf : tn, bz = tn > tf ? tn : tf;
+    const float t0 = fmaxf(fmaxf(fmaxf(tmin, ax), ay), az);
+    const float t1 = fminf(fminf(fminf(tmax, bx), by), bz);
     return t0 <= t1;
 }
 

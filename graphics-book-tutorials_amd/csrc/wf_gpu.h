@@ -174,6 +174,65 @@ struct LdsNodes {
     }
 };
 
+// Wave-scheduled traversal. The per-ray sequence is exactly traversal_step's (same nodes, same
+// primitive tests, same order), split into two phases so that a wave runs its expensive leaf
+// tests together: visit() moves a lane through one node and, when the node is a hit leaf, parks
+// the leaf in `leaf` (code first << 2 | count - 1) with the continuation already in tr.node;
+// leaf() tests a parked leaf's primitives. A lane is live while tr.node >= 0 or leaf >= 0.
+template <class Src>
+__device__ __forceinline__ void trav_visit(int n_nodes, const Src& src, Traversal& tr, int& leaf) {
+    const Node node = src.node(tr.node);
+    int n;
+    if (hit_box(node.lo, node.hi, tr.r.o, tr.inv_d, tr.r.tmin, tr.r.tmax)) {
+        if (node.child < 0) {
+            leaf = -(node.child + 1);
+            n = node.skip;
+        } else {
+            n = node.child;
+        }
+    } else {
+        n = node.skip;
+    }
+    tr.node = (n >= 0 && n < n_nodes) ? n : -1;
+}
+
+template <class Src>
+__device__ __forceinline__ void trav_leaf(const Src& src, Traversal& tr, int& leaf, bool anyhit) {
+    const int first = leaf >> 2, cnt = (leaf & 3) + 1;
+    leaf = -1;
+    for (int i = 0; i < cnt; ++i) {
+        float t;
+        if (src.hit(first + i, tr.r, &t)) {
+            tr.r.tmax = t;
+            tr.prim = first + i;
+            if (anyhit) {
+                tr.node = -1;
+                return;
+            }
+        }
+    }
+}
+
+// Runs the wave's live lanes until fewer than `limit` of them are live (limit 1: until all are
+// done). Each round either visits one node in every lane that has no parked leaf, or, once at
+// least `leaf_batch` lanes (or every live lane) have parked a leaf, tests those leaves together.
+template <class Src>
+__device__ __forceinline__ void wave_traverse(int n_nodes, const Src& src, Traversal& tr, int& leaf, bool anyhit,
+                                              int limit, int leaf_batch) {
+    for (;;) {
+        const bool parked = leaf >= 0;
+        const bool moving = tr.node >= 0 && !parked;
+        const int np = __popcll(__ballot(parked));
+        const int nm = __popcll(__ballot(moving));
+        if (np + nm < limit || np + nm == 0) break;
+        if (np > 0 && (nm == 0 || np >= leaf_batch)) {
+            if (parked) trav_leaf(src, tr, leaf, anyhit);
+        } else {
+            if (moving) trav_visit(n_nodes, src, tr, leaf);
+        }
+    }
+}
+
 // `lds` and `lds_prims` must point into LDS (__shared__ memory).
 __device__ __forceinline__ LdsNodes lds_nodes_of(const SceneView& s, const Node* lds, int n_lds,
                                                  const float4* lds_prims = nullptr) {

@@ -15,7 +15,9 @@
 // shade is a contiguous run. Traversal kernels run "while-while" with dynamic fetch: every wave
 // owns a contiguous range of rays and a lane whose ray has finished takes the next one as soon as
 // fewer than `refill` lanes of the wave are still traversing, so the wave does not idle on its
-// longest ray. The traversal step itself is the shared one, so hits are the megakernel's bits.
+// longest ray. Inside the loop a lane that reaches a hit leaf parks it, and the wave tests parked
+// leaves together once `leaf_batch` lanes have one; each ray still visits the nodes and tests the
+// primitives of the shared traversal in the same order, so hits are the megakernel's bits.
 // Every stage kernel reads its item count from the device control block, so an iteration needs
 // no host round trip; K iterations are captured once into a hipGraph and replayed until the control
 // block reports that all admitted work is done.
@@ -54,6 +56,8 @@ struct PersistentTuning {
     int lds_kib = 0;       // LDS per workgroup for BVH nodes and primitive records (0 -> 64 KiB)
     int graph_iters = 0;   // iterations per captured graph (0 -> 16; rounded up to even)
     int refill = 0;        // traversal refill threshold in lanes of 64 (0 -> 32)
+    int leaf_batch = 0;    // parked leaves that trigger a wave's leaf-test round (0 -> 16)
+    int dir_bins = 0;      // survivor grouping per block round: 1 none, 8 direction octants (0 -> 1)
 };
 
 struct PersistentResult {
@@ -133,7 +137,9 @@ public:
     void run_profiled(StageTimes* out);
     PersistentProgress progress();
     bool job_complete(int job) const;
-    void finish_job(int job);
+    // keep_buffer: keep the sample buffer for the next admitted job of a similar size instead of
+    // freeing it (render() does; a caller that budgets device memory itself does not).
+    void finish_job(int job, bool keep_buffer = false);
     void reset_jobs();
     uint64_t pool_bytes() const { return pool_bytes_; }
     uint64_t bytes() const;

@@ -20,7 +20,9 @@ constexpr int kDefaultPool = 1 << 22;
 constexpr int kDefaultBlock = 512;
 constexpr int kDefaultGraphIters = 16;
 constexpr int kDefaultLdsKiB = 64;
-constexpr int kDefaultRefill = 32;
+constexpr int kDefaultRefill = 16;
+constexpr int kDefaultLeafBatch = 16;
+constexpr int kDefaultDirBins = 1;
 constexpr uint32_t kIdle = 0xFFFFFFFFu;
 constexpr uint32_t kFinalFlag = 0x80000000u;
 
@@ -35,9 +37,10 @@ struct PCtx {
     DevCounters* ctr;
     uint32_t cap;
     int refill;
+    int leaf_batch;
+    int dir_bins;         // 1, or 8: survivors of a block round are grouped by direction octant
     LdsPlan plan0;        // single mode: the LDS layout
     SceneView scene0;     // single mode: the scene
-    JobDev job0;          // single mode: the job
 };
 
 template <bool SINGLE>
@@ -46,9 +49,10 @@ __device__ __forceinline__ const SceneView& scene_of(const PCtx& c, int sc) {
     else return c.scenes[sc];
 }
 
+// Single mode reads job 0 (a uniform address), so a captured graph does not depend on the job.
 template <bool SINGLE>
 __device__ __forceinline__ const JobDev& job_of(const PCtx& c, uint32_t jid) {
-    if constexpr (SINGLE) return c.job0;
+    if constexpr (SINGLE) return c.jobs[0];
     else return c.jobs[jid];
 }
 
@@ -212,6 +216,7 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
     Traversal tr;
     tr.node = -1;
     tr.prim = -1;
+    int leaf = -1;
     for (;;) {
         const uint64_t idle = __ballot(item == kIdle);
         if (idle != 0 && cursor < end) {
@@ -241,13 +246,8 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
             cursor = min(cursor + (uint32_t)__popcll(idle), end);
         }
         if (__ballot(item != kIdle) == 0) break;
-        const int limit = cursor < end ? c.refill : 1;
-        for (;;) {
-            const bool busy = tr.node >= 0;
-            if (__popcll(__ballot(busy)) < limit) break;
-            if (busy) traversal_step(nn, src, tr, false);
-        }
-        if (item != kIdle && tr.node < 0) {
+        wave_traverse(nn, src, tr, leaf, false, cursor < end ? c.refill : 1, c.leaf_batch);
+        if (item != kIdle && tr.node < 0 && leaf < 0) {
             Hit h{tr.r.tmax, tr.prim};
             if (!hit_is_valid(scene_of<SINGLE>(c, sc), h)) {
                 inv += 1;
@@ -260,11 +260,13 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
     wave_add_u32(&c.ctr->invalid_hits, inv);
 }
 
-// Classify + shade of items [0, it_act). Survivors are compacted into `out` in item order; slots
-// for survivors and shadow rays are reserved once per block round.
+// Classify + shade of items [0, it_act). Survivors are compacted into `out`; slots for survivors
+// and shadow rays are reserved once per block round. Inside a round the survivors are grouped by
+// the octant of their new direction (dir_bins = 8), each group in item order, so the waves of the
+// next extend see rays of nearby pixels heading the same way.
 template <bool SINGLE>
 __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, PathBuf out) {
-    __shared__ uint32_t cnt[2][3][16];
+    __shared__ uint32_t cnt[2][10][16];   // per round parity: 8 direction bins, shadow, finished
     __shared__ uint32_t bases[2][2];
     PState* st = c.st;
     const uint32_t n = st->it_act, n_live = st->it_live, n_jobs = st->n_jobs;
@@ -314,21 +316,34 @@ __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, Path
             fin = !live && !has;
             if (!live) job.sbuf[sidx] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
         }
-        const uint64_t ml = __ballot(live), ms = __ballot(has != 0), mf = __ballot(fin);
+        const int nb = c.dir_bins;
+        int bin = 0;
+        if (nb > 1)
+            bin = (ps.ray.d.x < 0.0f ? 1 : 0) | (ps.ray.d.y < 0.0f ? 2 : 0) | (ps.ray.d.z < 0.0f ? 4 : 0);
         uint32_t(*cn)[16] = cnt[round];
+        uint64_t ml = 0;
+        for (int b = 0; b < nb; ++b) {
+            const uint64_t m = __ballot(live && bin == b);
+            if (bin == b) ml = m;
+            if (lane_id() == 0) cn[b][wave] = (uint32_t)__popcll(m);
+        }
+        const uint64_t ms = __ballot(has != 0), mf = __ballot(fin);
         if (lane_id() == 0) {
-            cn[0][wave] = (uint32_t)__popcll(ml);
-            cn[1][wave] = (uint32_t)__popcll(ms);
-            cn[2][wave] = (uint32_t)__popcll(mf);
+            cn[8][wave] = (uint32_t)__popcll(ms);
+            cn[9][wave] = (uint32_t)__popcll(mf);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
             uint32_t t0 = 0, t1 = 0, t2 = 0;
+            for (int b = 0; b < nb; ++b)
+                for (uint32_t w = 0; w < nwaves; ++w) {
+                    const uint32_t a = cn[b][w];
+                    cn[b][w] = t0;
+                    t0 += a;
+                }
             for (uint32_t w = 0; w < nwaves; ++w) {
-                const uint32_t a = cn[0][w], b = cn[1][w], d = cn[2][w];
-                cn[0][w] = t0;
-                cn[1][w] = t1;
-                t0 += a;
+                const uint32_t b = cn[8][w], d = cn[9][w];
+                cn[8][w] = t1;
                 t1 += b;
                 t2 += d;
             }
@@ -340,7 +355,7 @@ __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, Path
         __syncthreads();
         uint32_t o = 0;
         if (live) {
-            o = bases[round][0] + cn[0][wave] + mbcnt64(ml);
+            o = bases[round][0] + cn[bin][wave] + mbcnt64(ml);
             out.ro[o] = make_float4(ps.ray.o.x, ps.ray.o.y, ps.ray.o.z, ps.prev_pdf);
             out.rd[o] = make_float4(ps.ray.d.x, ps.ray.d.y, ps.ray.d.z,
                                     __int_as_float(ps.depth | (ps.specular << 16)));
@@ -349,7 +364,7 @@ __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, Path
             out.kw[o] = make_uint4(key.k0, key.k1, jid, sidx);
         }
         if (has) {
-            const uint32_t e = bases[round][1] + cn[1][wave] + mbcnt64(ms);
+            const uint32_t e = bases[round][1] + cn[8][wave] + mbcnt64(ms);
             c.SQ.o[e] = make_float4(sh.o.x, sh.o.y, sh.o.z, sh.tmax);
             c.SQ.d[e] = make_float4(sh.d.x, sh.d.y, sh.d.z, __int_as_float((int)(jid | (live ? 0u : kFinalFlag))));
             c.SQ.c[e] = make_float4(contrib.x, contrib.y, contrib.z, __int_as_float((int)(live ? o : sidx)));
@@ -380,6 +395,7 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
     Traversal tr;
     tr.node = -1;
     tr.prim = -1;
+    int leaf = -1;
     for (;;) {
         const uint64_t idle = __ballot(item == kIdle);
         if (idle != 0 && cursor < end) {
@@ -404,15 +420,10 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
             cursor = min(cursor + (uint32_t)__popcll(idle), end);
         }
         if (__ballot(item != kIdle) == 0) break;
-        const int limit = cursor < end ? c.refill : 1;
-        for (;;) {
-            const bool busy = tr.node >= 0;
-            if (__popcll(__ballot(busy)) < limit) break;
-            if (busy) traversal_step(nn, src, tr, true);
-        }
+        wave_traverse(nn, src, tr, leaf, true, cursor < end ? c.refill : 1, c.leaf_batch);
         bool fin = false;
         uint32_t jid = 0;
-        if (item != kIdle && tr.node < 0) {
+        if (item != kIdle && tr.node < 0 && leaf < 0) {
             fin = (tag & kFinalFlag) != 0;
             if constexpr (!SINGLE) jid = tag & ~kFinalFlag;
             if (tr.prim < 0) {
@@ -464,6 +475,8 @@ struct PersistentEngine::Impl {
     DevCounters* ctr = nullptr;
     int block = kDefaultBlock, graph_iters = kDefaultGraphIters, lds_bytes = kDefaultLdsKiB * 1024, cus = 0;
     int refill = kDefaultRefill;
+    int leaf_batch = kDefaultLeafBatch;
+    int dir_bins = kDefaultDirBins;
     std::vector<void*> pool_allocs;
     PState* st = nullptr;
     PState* st_host = nullptr;       // pinned mirror
@@ -484,16 +497,36 @@ struct PersistentEngine::Impl {
     unsigned long long total_work = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    bool graph_single = false;
-    int graph_scenes = -1;
+    // What a captured graph depends on: single/multi mode, the scene table (baked in single mode,
+    // and the LDS size of every launch), the launch shape and scheduling parameters.
+    struct GraphKey {
+        bool single = false;
+        uint64_t scene_gen = 0;
+        int block = 0, refill = 0, leaf_batch = 0, dir_bins = 0, cap = 0, iters = 0;
+        size_t lds_bytes = 0;
+        const void* ctr = nullptr;
+        bool operator==(const GraphKey& o) const {
+            return single == o.single && scene_gen == o.scene_gen && block == o.block && refill == o.refill &&
+                   leaf_batch == o.leaf_batch && dir_bins == o.dir_bins && cap == o.cap && iters == o.iters && lds_bytes == o.lds_bytes &&
+                   ctr == o.ctr;
+        }
+    } graph_key;
+    uint64_t scene_gen = 1;   // bumped whenever the scene table changes
     uint64_t sbuf_bytes = 0;
+    // One finished job's sample buffer kept for the next job of a similar size (render() only).
+    float4* spare = nullptr;
+    uint64_t spare_bytes = 0;
 
     void free_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         exec = nullptr;
         graph = nullptr;
-        graph_scenes = -1;
+    }
+    void free_spare() {
+        if (spare) (void)hipFree(spare);
+        spare = nullptr;
+        spare_bytes = 0;
     }
     void free_pool() {
         for (void* p : pool_allocs) (void)hipFree(p);
@@ -501,6 +534,7 @@ struct PersistentEngine::Impl {
     }
     ~Impl() {
         free_graph();
+        free_spare();
         for (HostJob& j : hjobs)
             if (j.dev.sbuf) (void)hipFree(j.dev.sbuf);
         free_pool();
@@ -519,7 +553,7 @@ struct PersistentEngine::LaunchPlan {
 PersistentEngine::PersistentEngine() : m_(std::make_unique<Impl>()) {}
 PersistentEngine::~PersistentEngine() = default;
 
-uint64_t PersistentEngine::bytes() const { return pool_bytes_ + m_->sbuf_bytes; }
+uint64_t PersistentEngine::bytes() const { return pool_bytes_ + m_->sbuf_bytes + m_->spare_bytes; }
 
 void PersistentEngine::ensure_pool(const PersistentTuning& t) {
     Impl& m = *m_;
@@ -535,17 +569,30 @@ void PersistentEngine::ensure_pool(const PersistentTuning& t) {
         throw std::runtime_error("persistent: block size must be 256, 512 or 1024");
     m.refill = t.refill > 0 ? t.refill : kDefaultRefill;
     if (m.refill > 64) throw std::runtime_error("persistent: refill threshold must be 1..64 lanes");
+    m.leaf_batch = t.leaf_batch > 0 ? t.leaf_batch : kDefaultLeafBatch;
+    if (m.leaf_batch > 64) throw std::runtime_error("persistent: leaf batch must be 1..64 lanes");
+    m.dir_bins = t.dir_bins > 0 ? t.dir_bins : kDefaultDirBins;
+    if (m.dir_bins != 1 && m.dir_bins != 8) throw std::runtime_error("persistent: direction bins must be 1 or 8");
     m.ctx.refill = m.refill;
+    m.ctx.leaf_batch = m.leaf_batch;
+    m.ctx.dir_bins = m.dir_bins;
     m.graph_iters = t.graph_iters > 0 ? ((t.graph_iters + 1) & ~1) : kDefaultGraphIters;
     const int kib = t.lds_kib > 0 ? t.lds_kib : kDefaultLdsKiB;
     if (kib > 64) throw std::runtime_error("persistent: at most 64 KiB of LDS per workgroup");
-    m.lds_bytes = kib * 1024;
-    if (cap == cap_ && !m.pool_allocs.empty()) {
-        m.free_graph();   // the captured graph bakes in the launch shape and refill threshold
-        return;
+    if (m.lds_bytes != kib * 1024) {
+        m.lds_bytes = kib * 1024;
+        if (!m.hscenes.empty()) {
+            update_lds_plan();
+            m.scene_gen += 1;
+        }
     }
+    if (cap == cap_ && !m.pool_allocs.empty()) return;
     m.free_graph();
     m.free_pool();
+    // The device scene table lives in the pool allocation.
+    m.scene_keys.clear();
+    m.hscenes.clear();
+    m.scene_gen += 1;
     pool_bytes_ = 0;
     cap_ = cap;
     auto alloc = [&](auto** p, size_t bytes) {
@@ -559,6 +606,8 @@ void PersistentEngine::ensure_pool(const PersistentTuning& t) {
     PCtx& c = m.ctx;
     c = PCtx{};
     c.refill = m.refill;
+    c.leaf_batch = m.leaf_batch;
+    c.dir_bins = m.dir_bins;
     for (PathBuf& b : m.S) {
         alloc(&b.ro, P * 16);
         alloc(&b.rd, P * 16);
@@ -596,15 +645,11 @@ void PersistentEngine::reset_jobs() {
     for (Impl::HostJob& j : m.hjobs)
         if (j.dev.sbuf) (void)hipFree(j.dev.sbuf);
     m.hjobs.clear();
-    m.hscenes.clear();
-    m.scene_keys.clear();
     m.sbuf_bytes = 0;
     m.total_work = 0;
-    m.free_graph();
     pw_init_kernel<<<1, 1, 0, m.stream>>>(m.st);
     m.par = 0;
     check(hipGetLastError(), "persistent init");
-    update_lds_plan();
 }
 
 // Splits the LDS budget evenly over the scenes: BVH nodes first (the breadth-first top), then the
@@ -641,18 +686,24 @@ int PersistentEngine::admit(const PersistentJob& job) {
     const uint64_t WH = (uint64_t)job.scene->width * job.scene->height;
     const uint64_t work = WH * (uint64_t)job.spp;
     if (work >= (1ull << 32)) throw std::runtime_error("persistent: job has 2^32 or more paths");
+    // Scene table: kept across job resets; an entry matches by address and by contents (a view
+    // rebuilt at the same address is a different scene).
     int sc = -1;
     for (size_t k = 0; k < m.scene_keys.size(); ++k)
-        if (m.scene_keys[k] == job.scene) sc = (int)k;
+        if (m.scene_keys[k] == job.scene && std::memcmp(&m.hscenes[k], job.scene, sizeof(SceneView)) == 0) sc = (int)k;
     if (sc < 0) {
-        if ((int)m.scene_keys.size() >= kMaxScenes) throw std::runtime_error("persistent: too many distinct scenes");
+        if ((int)m.scene_keys.size() >= kMaxScenes) {
+            if (!m.hjobs.empty()) throw std::runtime_error("persistent: too many distinct scenes");
+            m.scene_keys.clear();
+            m.hscenes.clear();
+        }
         sc = (int)m.scene_keys.size();
         m.scene_keys.push_back(job.scene);
         m.hscenes.push_back(*job.scene);
         check(hipMemcpyAsync(m.ctx.scenes + sc, job.scene, sizeof(SceneView), hipMemcpyHostToDevice, m.stream),
               "copy scene");
         update_lds_plan();
-        m.free_graph();
+        m.scene_gen += 1;
     }
     JobDev d{};
     d.scene = sc;
@@ -665,7 +716,15 @@ int PersistentEngine::admit(const PersistentJob& job) {
     d.work_begin = m.total_work;
     d.work_end = m.total_work + work;
     d.film = job.film;
-    check(hipMalloc(&d.sbuf, (size_t)work * sizeof(float4)), "hipMalloc sample buffer");
+    const uint64_t need = (uint64_t)work * sizeof(float4);
+    if (m.spare && m.spare_bytes >= need && m.spare_bytes <= 2 * need) {
+        d.sbuf = m.spare;
+        m.spare = nullptr;
+        m.spare_bytes = 0;
+    } else {
+        m.free_spare();
+        check(hipMalloc(&d.sbuf, (size_t)need), "hipMalloc sample buffer");
+    }
     m.sbuf_bytes += (uint64_t)work * sizeof(float4);
     m.total_work = d.work_end;
     const uint32_t index = (uint32_t)m.hjobs.size();
@@ -675,7 +734,6 @@ int PersistentEngine::admit(const PersistentJob& job) {
     hj.dev = d;
     hj.work = work;
     m.hjobs.push_back(hj);
-    if (m.graph && m.graph_single) m.free_graph();   // the single-job graph bakes the job in
     return (int)index;
 }
 
@@ -685,10 +743,7 @@ PersistentEngine::LaunchPlan PersistentEngine::make_plan(bool single) {
     L.single = single;
     L.c = m.ctx;
     L.c.plan0 = m.lds_plan;
-    if (single) {
-        L.c.scene0 = m.hscenes[0];
-        L.c.job0 = m.hjobs[0].dev;
-    }
+    if (single) L.c.scene0 = m.hscenes[0];
     L.lds_bytes = (size_t)m.lds_plan.total_nodes * sizeof(Node) + (size_t)m.lds_plan.total_prims * sizeof(float4);
     const void* fx = single ? (const void*)pw_extend_kernel<true> : (const void*)pw_extend_kernel<false>;
     const void* fs = single ? (const void*)pw_shade_kernel<true> : (const void*)pw_shade_kernel<false>;
@@ -731,16 +786,27 @@ void PersistentEngine::capture(bool single) {
     check(hipStreamEndCapture(m.stream, &m.graph), "hipStreamEndCapture");
     check(hipGetLastError(), "persistent capture");
     check(hipGraphInstantiate(&m.exec, m.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-    m.graph_single = single;
-    m.graph_scenes = (int)m.hscenes.size();
 }
 
 void PersistentEngine::run(int launches) {
     Impl& m = *m_;
     if (m.hjobs.empty()) return;
-    const bool single = m.hjobs.size() == 1 && m.hscenes.size() == 1;
-    if (!m.exec || m.graph_single != single || m.graph_scenes != (int)m.hscenes.size()) capture(single);
-    for (int k = 0; k < launches; ++k) check(hipGraphLaunch(m.exec, m.stream), "hipGraphLaunch");
+    Impl::GraphKey k;
+    k.single = m.hjobs.size() == 1 && m.hscenes.size() == 1;
+    k.scene_gen = m.scene_gen;
+    k.block = m.block;
+    k.refill = m.refill;
+    k.leaf_batch = m.leaf_batch;
+    k.dir_bins = m.dir_bins;
+    k.cap = cap_;
+    k.iters = m.graph_iters;
+    k.ctr = m.ctx.ctr;
+    k.lds_bytes = (size_t)m.lds_plan.total_nodes * sizeof(Node) + (size_t)m.lds_plan.total_prims * sizeof(float4);
+    if (!m.exec || !(k == m.graph_key)) {
+        capture(k.single);
+        m.graph_key = k;
+    }
+    for (int i = 0; i < launches; ++i) check(hipGraphLaunch(m.exec, m.stream), "hipGraphLaunch");
 }
 
 // Eager iterations with HIP events around each stage (synchronising once per iteration), until
@@ -814,16 +880,22 @@ bool PersistentEngine::job_complete(int job) const {
     return (uint64_t)j.done == j.work;
 }
 
-void PersistentEngine::finish_job(int job) {
+void PersistentEngine::finish_job(int job, bool keep_buffer) {
     Impl& m = *m_;
     Impl::HostJob& j = m.hjobs[job];
     if (j.finished) return;
     const int WH = j.dev.width * j.dev.height;
     pw_resolve_kernel<<<(WH + 255) / 256, 256, 0, m.stream>>>(j.dev);
     check(hipGetLastError(), "persistent resolve");
-    check(hipStreamSynchronize(m.stream), "persistent resolve sync");
-    check(hipFree(j.dev.sbuf), "hipFree sample buffer");
-    m.sbuf_bytes -= (uint64_t)WH * j.dev.spp * sizeof(float4);
+    const uint64_t bytes = (uint64_t)WH * j.dev.spp * sizeof(float4);
+    if (keep_buffer && !m.spare) {
+        m.spare = j.dev.sbuf;   // stream order: the next job writes it after this resolve
+        m.spare_bytes = bytes;
+    } else {
+        check(hipStreamSynchronize(m.stream), "persistent resolve sync");
+        check(hipFree(j.dev.sbuf), "hipFree sample buffer");
+    }
+    m.sbuf_bytes -= bytes;
     j.dev.sbuf = nullptr;
     j.finished = true;
 }
@@ -877,7 +949,7 @@ void PersistentEngine::render(const PersistentJob* jobs, int n_jobs, const Persi
         res.iterations += (int)pr.iterations;
         for (int k = 0; k < (int)m_->hjobs.size(); ++k) {
             if (!job_complete(k)) throw std::runtime_error("persistent: job incomplete after the work drained");
-            finish_job(k);
+            finish_job(k, true);
         }
         res.chunks += 1;
     }
@@ -917,29 +989,48 @@ PersistentTuning autotune_persistent(const PersistentJob* jobs, int n_jobs, hipS
             best = t;
         }
     };
-    // Stage 1: workgroup size x refill threshold at the largest pool and LDS budget.
+    // Coordinate search. Stage 1: workgroup size at the largest pool and LDS budget.
     const int pools[] = {1 << 20, 1 << 21, 1 << 22};
     int top = pools[0];
     for (int p : pools)
         if ((uint64_t)p <= 2 * work) top = p;
-    for (int bs : {256, 512, 1024})
-        for (int rf : {16, 32, 48}) {
-            PersistentTuning t;
-            t.block_size = bs;
+    for (int bs : {256, 512, 1024}) {
+        PersistentTuning t;
+        t.block_size = bs;
+        t.refill = kDefaultRefill;
+        t.leaf_batch = kDefaultLeafBatch;
+        t.dir_bins = kDefaultDirBins;
+        t.pool_paths = top;
+        t.lds_kib = 64;
+        time(t);
+    }
+    // Stage 2: traversal scheduling (refill threshold x leaf batch) for that workgroup size, then
+    // survivor grouping.
+    PersistentTuning base = best;
+    for (int rf : {8, 16, 32})
+        for (int lb : {4, 8, 16, 32}) {
+            if (rf == base.refill && lb == base.leaf_batch) continue;
+            PersistentTuning t = base;
             t.refill = rf;
-            t.pool_paths = top;
-            t.lds_kib = 64;
+            t.leaf_batch = lb;
             time(t);
         }
-    // Stage 2: pool size (queue capacity) and LDS budget for the best shape.
-    const PersistentTuning shape = best;
+    base = best;
+    for (int nb : {1, 8}) {
+        if (nb == (base.dir_bins > 0 ? base.dir_bins : kDefaultDirBins)) continue;
+        PersistentTuning t = base;
+        t.dir_bins = nb;
+        time(t);
+    }
+    // Stage 3: pool size (queue capacity) and LDS budget.
+    base = best;
     for (int p : pools) {
         if (p > top) continue;
         for (int kib : {32, 64}) {
-            PersistentTuning t = shape;
+            if (p == base.pool_paths && kib == base.lds_kib) continue;
+            PersistentTuning t = base;
             t.pool_paths = p;
             t.lds_kib = kib;
-            if (p == shape.pool_paths && kib == shape.lds_kib) continue;
             time(t);
         }
     }

@@ -26,6 +26,8 @@ struct RenderParams {
     int scan_tile = 0;           // plain wavefront compaction tile
     int lds_kib = 0;             // persistent: LDS per workgroup (KiB)
     int refill = 0;              // persistent: traversal refill threshold (lanes)
+    int leaf_batch = 0;          // persistent: parked leaves per leaf-test round
+    int dir_bins = 0;            // persistent: survivor grouping (1 or 8 direction octants)
     int lds_nodes = -1;          // nodes staged in LDS per workgroup (-1 = default)
 };
 

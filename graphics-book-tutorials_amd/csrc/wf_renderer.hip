@@ -146,6 +146,8 @@ void GpuRenderer::run_passes(int first_sample, int count, const RenderParams& p,
             tune.pool_paths = p.pool_paths;
             tune.lds_kib = p.lds_kib;
             tune.refill = p.refill;
+            tune.leaf_batch = p.leaf_batch;
+            tune.dir_bins = p.dir_bins;
             gpu::PersistentResult pr;
             m.pe->render(&job, 1, tune, m.d_ctr, m.stream, &pr, p.stage_timing ? &st : nullptr);
             iterations = pr.iterations;

@@ -81,6 +81,13 @@ __global__ void trace_kernel(SceneView s, const int* pxys, int n, uint32_t seed,
     rays[2 * i + 1] = rc.shadow;
 }
 
+// Plain 16-byte-per-lane copy: the calibration kernel for hardware traffic counters (a kernel
+// whose device-memory bytes are known exactly: n16 * 16 read, n16 * 16 written).
+__global__ void copy_probe_kernel(const float4* __restrict__ src, float4* __restrict__ dst, size_t n16) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 template <class T>
 struct DevBuf {
     T* p = nullptr;
@@ -239,6 +246,33 @@ int wf_test_trace_paths(const wf_scene* scene, const int* pxys, int n, uint32_t 
         check(hipDeviceSynchronize(), "trace sync");
         dl.get(L, (size_t)3 * n);
         dc.get(rays, (size_t)2 * n);
+    });
+}
+
+int wf_copy_probe(uint64_t bytes, int reps, double* ms_per_copy) {
+    return hook([&] {
+        if (bytes < 16 || bytes % 16 != 0 || reps < 1 || !ms_per_copy) throw std::invalid_argument("bad argument");
+        const size_t n16 = bytes / 16;
+        DevBuf<float4> src(n16), dst(n16);
+        check(hipMemset(src.p, 0, bytes), "hipMemset");
+        int dev = 0, cus = 0;
+        check(hipGetDevice(&dev), "hipGetDevice");
+        check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+        const int grid = cus * 8;
+        copy_probe_kernel<<<grid, 256>>>(src.p, dst.p, n16);   // warm-up
+        hipEvent_t a, b;
+        check(hipEventCreate(&a), "hipEventCreate");
+        check(hipEventCreate(&b), "hipEventCreate");
+        check(hipEventRecord(a, nullptr), "hipEventRecord");
+        for (int k = 0; k < reps; ++k) copy_probe_kernel<<<grid, 256>>>(src.p, dst.p, n16);
+        check(hipEventRecord(b, nullptr), "hipEventRecord");
+        check(hipEventSynchronize(b), "hipEventSynchronize");
+        check(hipGetLastError(), "copy probe");
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+        *ms_per_copy = (double)ms / reps;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
     });
 }
 

@@ -33,7 +33,8 @@ class RenderParams(C.Structure):
     _fields_ = [("spp", C.c_int), ("seed", C.c_uint32), ("max_depth", C.c_int), ("threads", C.c_int),
                 ("count_rays", C.c_int), ("stage_timing", C.c_int), ("queue_capacity", C.c_int),
                 ("trace_bounces", C.c_int), ("block_size", C.c_int), ("pool_paths", C.c_int),
-                ("scan_tile", C.c_int), ("lds_nodes", C.c_int), ("lds_kib", C.c_int), ("refill", C.c_int)]
+                ("scan_tile", C.c_int), ("lds_nodes", C.c_int), ("lds_kib", C.c_int), ("refill", C.c_int),
+                ("leaf_batch", C.c_int), ("dir_bins", C.c_int)]
 
 
 class RenderStats(C.Structure):
@@ -92,7 +93,8 @@ class BatchStats(C.Structure):
     _fields_ = [("render_ms", C.c_double), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("invalid_hits", C.c_int64), ("queue_overflows", C.c_int64), ("iterations", C.c_int64),
                 ("graph_launches", C.c_int64), ("block_size", C.c_int), ("pool_paths", C.c_int),
-                ("lds_kib", C.c_int), ("refill", C.c_int), ("device_bytes", C.c_uint64)]
+                ("lds_kib", C.c_int), ("refill", C.c_int), ("leaf_batch", C.c_int),
+                ("dir_bins", C.c_int), ("device_bytes", C.c_uint64)]
 
 
 class ServerStats(C.Structure):
@@ -115,7 +117,7 @@ EXPORTS = [
     "wf_scan_workspace_destroy", "wf_scan_exclusive_u32", "wf_compact_u32", "wf_test_queue_push",
     "wf_batch_render", "wf_autotune", "wf_server_create", "wf_server_destroy", "wf_server_submit",
     "wf_server_step", "wf_server_poll", "wf_server_stats_get", "wf_test_sincos", "wf_test_sincos_grid_hash",
-    "wf_test_intersect", "wf_test_trace_paths",
+    "wf_test_intersect", "wf_test_trace_paths", "wf_copy_probe",
 ]
 
 
@@ -191,9 +193,10 @@ def _iptr(a):
 
 
 def params(spp=64, seed=1234, max_depth=8, threads=0, stage_timing=False, queue_capacity=0, trace_bounces=False,
-           block_size=0, pool_paths=0, scan_tile=0, lds_nodes=-1, lds_kib=0, refill=0):
+           block_size=0, pool_paths=0, scan_tile=0, lds_nodes=-1, lds_kib=0, refill=0, leaf_batch=0,
+           dir_bins=0):
     return RenderParams(spp, seed, max_depth, threads, 0, int(stage_timing), queue_capacity, int(trace_bounces),
-                        block_size, pool_paths, scan_tile, lds_nodes, lds_kib, refill)
+                        block_size, pool_paths, scan_tile, lds_nodes, lds_kib, refill, leaf_batch, dir_bins)
 
 
 class Scene:
@@ -368,6 +371,13 @@ def query_device():
     return dict(name=f.name.decode(), arch=f.arch.decode(), compute_units=f.compute_units,
                 runtime_version=f.runtime_version, driver_version=f.driver_version,
                 total_memory_bytes=f.total_memory_bytes, free_memory_bytes=f.free_memory_bytes)
+
+
+def copy_probe(nbytes, reps=10):
+    """Average ms of a plain device copy of nbytes (reads and writes nbytes; counter calibration)."""
+    ms = C.c_double()
+    _check(lib().wf_copy_probe(C.c_uint64(nbytes), reps, C.byref(ms)))
+    return ms.value
 
 
 def test_sincos(x, device=False):

@@ -53,6 +53,8 @@ typedef struct wf_render_params {
     int lds_nodes;        /* megakernel / plain wavefront: BVH nodes staged in LDS, -1 = default */
     int lds_kib;          /* persistent: LDS per workgroup for BVH nodes + primitive records (KiB) */
     int refill;           /* persistent: a wave refetches rays when fewer lanes than this traverse */
+    int leaf_batch;       /* persistent: parked leaves that start a wave's leaf-test round */
+    int dir_bins;         /* persistent: survivor grouping, 1 none or 8 direction octants */
 } wf_render_params;
 
 /* RenderStats, include/wf/renderer.hpp:23-37. stage_ms: extend, classify, shade_lambert,
@@ -190,6 +192,12 @@ int wf_test_intersect(const wf_scene* scene, const float* rays, int n, int anyhi
 int wf_test_trace_paths(const wf_scene* scene, const int* pxys, int n, uint32_t seed, int max_depth, float* L,
                         uint32_t* rays, int device);
 
+/* ---- measurement (no reference counterpart) ---- */
+/* Runs a plain 16-byte-per-lane device copy of `bytes` (multiple of 16) reps times and returns the
+ * average duration per copy. It reads and writes exactly `bytes`, so hardware traffic counters
+ * (FETCH_SIZE, WRITE_SIZE) read on it calibrate the counters read on the engine's kernels. */
+int wf_copy_probe(uint64_t bytes, int reps, double* ms_per_copy);
+
 /* ---- batch: many jobs in one persistent launch (no reference counterpart) ---- */
 typedef struct wf_job {
     const wf_scene* scene;
@@ -205,7 +213,7 @@ typedef struct wf_batch_stats {
     uint64_t extension_rays, shadow_rays;
     int64_t invalid_hits, queue_overflows;
     int64_t iterations, graph_launches;
-    int block_size, pool_paths, lds_kib, refill;
+    int block_size, pool_paths, lds_kib, refill, leaf_batch, dir_bins;
     uint64_t device_bytes;
 } wf_batch_stats;
 
