@@ -25,6 +25,14 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 METRIC = "Mrays/s at 1280x720x64spp depth 8 per scene; p50/p99 render latency"
 HBM_PEAK_GBS = 8000.0
+# Device-memory bytes per launch with work measured by rocprofv3 FETCH_SIZE and WRITE_SIZE on this
+# workload (sphere_grid 1280x720x64, default tuning), each counter scaled by what it reads on a plain
+# copy kernel of known size (FETCH_SIZE x2.0, WRITE_SIZE x1.0); source and commands:
+# profiles/persistent_sphere_grid_1280x720x64_counters.md. Valid for that workload only.
+MEASURED_TRAFFIC = {"sphere_grid 1280x720x64": {"extend": 94.5e6, "shade": 399.8e6, "shadow": 56.6e6}}
+# VALU lane utilisation of the traversal kernels (same source).
+MEASURED_LANE_UTIL = {"sphere_grid 1280x720x64": {"extend": 0.605, "shadow": 0.426, "shade": 0.470,
+                                                   "megakernel": 0.193}}
 
 
 def percentile(v, p):
@@ -37,15 +45,26 @@ def percentile(v, p):
     return v[i] + (v[min(i + 1, len(v) - 1)] - v[i]) * f
 
 
-def cpu_baseline(wfamd, scene_name, w, h, spp_sample):
+def cpu_baseline(scene, scene_name, w, h, spp_sample, gpu_sums, runs=3):
+    """The oracle (a plain-C restatement of the same estimator) rendering the first spp_sample
+    samples of the workload on the host cores: median wall time of `runs` renders. Its image is
+    also the CPU side of the quality gate: the GPU's render of the same samples must have at least
+    99.9% of its pixels within 3 combined standard errors of it (reported with the fraction of
+    bitwise-equal pixels)."""
     import numpy as np
     import pyoracle
     threads = max(1, min(16, os.cpu_count() or 1))
-    s = wfamd.Scene(scene_name, w, h)
-    o = pyoracle.OScene(s.arrays())
-    t0 = time.perf_counter()
-    _, cnt = o.render_sums(0, spp_sample, 1234, 8, threads=threads)
-    dt = time.perf_counter() - t0
+    o = pyoracle.OScene(scene.arrays())
+    times, sums, cnt = [], None, None
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        sums, cnt = o.render_sums(0, spp_sample, 1234, 8, threads=threads)
+        times.append(time.perf_counter() - t0)
+    dt = percentile(times, 0.5)
+    rgb_c, var_c = pyoracle.finalize(*sums, spp_sample)
+    rgb_g, var_g = pyoracle.finalize(*gpu_sums, spp_sample)
+    cmp = pyoracle.compare(rgb_g, var_g, spp_sample, rgb_c, var_c, spp_sample)
+    same = (rgb_g.view(np.uint32).reshape(-1, 3) == rgb_c.view(np.uint32).reshape(-1, 3)).all(axis=1)
     model = ""
     try:
         for line in open("/proc/cpuinfo"):
@@ -54,16 +73,21 @@ def cpu_baseline(wfamd, scene_name, w, h, spp_sample):
                 break
     except OSError:
         pass
-    return {"value": (cnt[0] + cnt[1]) / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{scene_name} {w}x{h}, {spp_sample} spp (of 64), depth 8, seed 1234; {cnt[0]} extension + "
-                      f"{cnt[1]} shadow rays in {dt:.2f} s; {model}"}
+    base = {"value": (cnt[0] + cnt[1]) / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{scene_name} {w}x{h}, samples 0..{spp_sample - 1} of 64, depth 8, seed 1234: {cnt[0]} "
+                      f"extension + {cnt[1]} shadow rays, median of {runs} renders {dt:.3f} s "
+                      f"({', '.join(f'{t:.3f}' for t in times)}); {model}"}
+    gate = {"frac_within_3se": cmp["frac_within_3se"], "threshold": 0.999,
+            "pass": cmp["frac_within_3se"] >= 0.999 and cmp["nonfinite"] == 0,
+            "bitwise_pixel_fraction": float(same.mean()), "samples": spp_sample}
+    return base, gate
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scene", default="sphere_grid")
     ap.add_argument("--width", type=int, default=1280)
     ap.add_argument("--height", type=int, default=720)
@@ -120,6 +144,12 @@ def main():
     print(f"[bench] persistent: {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
     sums_p = r.download_sums()
     rgb_p, var_p = wfamd.finalize(*sums_p, spp)
+    # The boundary hands the image back to the host: time that download (mean rgb + luminance
+    # variance, 16 B per pixel over PCIe) beside the render for the PCIe-inclusive rate.
+    t1 = time.perf_counter()
+    for _ in range(5):
+        r.download(spp)
+    download_ms = (time.perf_counter() - t1) / 5 * 1e3
 
     # Megakernel on the same workload: throughput beside the flagship, and the bitwise gate.
     mk = wfamd.Renderer(scene, "megakernel")
@@ -156,8 +186,13 @@ def main():
         launches = max(d["launches"], 1)
         avg_s = d["ms"] / 1e3 / launches
         achieved = d["bytes"] / launches / avg_s / 1e9
+        wl = f"{args.scene} {args.width}x{args.height}x{spp}"
+        traffic = MEASURED_TRAFFIC.get(wl, {}).get(dom)
         roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "note": ("extend and shadow are BVH traversal: bound by LDS-latency node visits and lane "
+                         "divergence, not HBM; their VALU lane utilisation is given in lane_util"),
+                "lane_util": MEASURED_LANE_UTIL.get(wl),
                 "avg_launch_us": avg_s * 1e6, "bytes_per_launch": d["bytes"] / launches,
                 "items_per_launch": d["items"] / launches,
                 "stages": {k: {"ms": v["ms"], "launches": v["launches"], "items": v["items"],
@@ -165,7 +200,15 @@ def main():
                            for k, v in stages.items()}}
     r.close()
 
-    cpu = None if args.no_cpu else cpu_baseline(wfamd, args.scene, args.width, args.height, args.cpu_spp)
+    cpu, cpu_gate = None, None
+    if not args.no_cpu:
+        # GPU side of the CPU quality gate: the same first cpu_spp samples on the persistent engine.
+        r = wfamd.Renderer(scene, "persistent")
+        r.reset()
+        r.run_passes(0, args.cpu_spp, tuning)
+        gpu_sample = r.download_sums()
+        r.close()
+        cpu, cpu_gate = cpu_baseline(scene, args.scene, args.width, args.height, args.cpu_spp, gpu_sample)
     value = (ext + sh) / elapsed / 1e6
     out = {
         "metric": METRIC, "value": value, "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -179,12 +222,15 @@ def main():
                    "parallelism": "single GPU"},
         "p50_ms": percentile(lat, 0.5), "p99_ms": percentile(lat, 0.99),
         "extension_mrays_per_s": ext / elapsed / 1e6, "rays_per_render": rays_per_render,
+        "download_ms": download_ms,
+        "mrays_per_s_with_download": rays_per_render / ((elapsed / args.steps * 1e3 + download_ms) * 1e3),
         "megakernel": {"mrays_per_s": mk_mrays, "p50_ms": percentile(mk_lat, 0.5)},
         "speedup_vs_megakernel": value / mk_mrays if mk_mrays else None,
         "quality": {"persistent_equals_megakernel_bitwise": bool(bitwise),
                     "queue_overflows": int(last.queue_overflows), "invalid_hits": int(last.invalid_hits),
                     "nonfinite_pixels": int((~np.isfinite(rgb_p.reshape(-1, 3))).any(axis=1).sum()),
-                    "nonfinite_var": int((~np.isfinite(var_p)).sum())},
+                    "nonfinite_var": int((~np.isfinite(var_p)).sum()),
+                    "cpu_vs_persistent_within_3se": cpu_gate},
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
