@@ -252,6 +252,20 @@ int wf_scene_orbit(wf_scene* scene, float yaw_deg) {
     });
 }
 
+int wf_scene_view(const wf_scene* scene, int width, int height, float yaw_deg, int orbit, wf_scene** out) {
+    return guard([&] {
+        if (!scene || !out) throw std::invalid_argument("null argument");
+        if (width < 1 || height < 1 || width > 8192 || height > 8192) throw std::invalid_argument("image size outside 1..8192");
+        auto s = std::make_unique<wf_scene>();
+        s->s = scene->s;
+        s->s.width = width;
+        s->s.height = height;
+        if (orbit) wf::orbit_camera(s->s, yaw_deg);
+        *out = s.release();
+        return (int)WF_OK;
+    });
+}
+
 void wf_scene_destroy(wf_scene* scene) { delete scene; }
 
 int wf_scene_info_get(const wf_scene* scene, wf_scene_info* out) {
@@ -603,6 +617,18 @@ int wf_test_queue_push(const uint32_t* d_flags, int n, uint32_t* d_items, uint32
 // ---------------------------------------------------------------------------------------------
 namespace {
 
+// Size and camera rig of a job's view (validated), filled into the engine job.
+void job_view(const wf_job& in, wf::gpu::PersistentJob* j, int* w, int* h) {
+    const wf::Scene& sc = in.scene->s;
+    *w = in.width > 0 ? in.width : sc.width;
+    *h = in.height > 0 ? in.height : sc.height;
+    if (in.width < 0 || in.height < 0 || *w > 8192 || *h > 8192) throw std::invalid_argument("job image size outside 1..8192");
+    j->has_view = true;
+    j->width = *w;
+    j->height = *h;
+    j->cam = wf::view_rig(sc, *w, *h, in.orbit != 0, in.yaw_deg);
+}
+
 wf::gpu::PersistentTuning to_tuning(const wf_render_params* p) {
     wf::gpu::PersistentTuning t;
     if (p) {
@@ -637,8 +663,10 @@ struct BatchSetup {
             if (in[i].spp < 1) throw std::invalid_argument("job spp must be >= 1");
             if (in[i].max_depth < 0 || in[i].max_depth > 1024) throw std::invalid_argument("job max_depth out of range");
             const wf::SceneView* v = scenes.get(in[i].scene);
-            films[i].alloc(v->width * v->height, stream);
             wf::gpu::PersistentJob j;
+            int w = 0, h = 0;
+            job_view(in[i], &j, &w, &h);
+            films[i].alloc(w * h, stream);
             j.scene = v;
             j.film = films[i].film;
             j.first_sample = 0;
@@ -743,7 +771,8 @@ struct wf_server {
     }
 
     uint64_t request_bytes(const wf_job& j) const {
-        const uint64_t px = (uint64_t)j.scene->s.width * j.scene->s.height;
+        const uint64_t px = (uint64_t)(j.width > 0 ? j.width : j.scene->s.width) *
+                            (uint64_t)(j.height > 0 ? j.height : j.scene->s.height);
         return px * 5 * sizeof(float) + px * (uint64_t)j.spp * sizeof(float4);
     }
 };
@@ -773,6 +802,8 @@ int wf_server_submit(wf_server* s, const wf_job* job, int64_t* id) {
     return guard([&] {
         if (!s || !job || !job->scene || !id) throw std::invalid_argument("null argument");
         if (job->spp < 1 || job->max_depth < 0 || job->max_depth > 1024) throw std::invalid_argument("bad job");
+        if (job->width < 0 || job->height < 0 || job->width > 8192 || job->height > 8192)
+            throw std::invalid_argument("job image size outside 1..8192");
         s->submitted++;
         const uint64_t need = s->request_bytes(*job) + s->scenes.bytes();
         if (s->eng.pool_bytes() + need > s->budget) {
@@ -807,8 +838,10 @@ int wf_server_step(wf_server* s, int launches) {
             const uint64_t scene_before = s->scenes.bytes();
             const wf::SceneView* v = s->scenes.get(r->job.scene);
             s->in_use += s->scenes.bytes() - scene_before;   // scene uploads stay resident
-            r->film.alloc(v->width * v->height, s->stream);
             wf::gpu::PersistentJob j;
+            int w = 0, h = 0;
+            job_view(r->job, &j, &w, &h);
+            r->film.alloc(w * h, s->stream);
             j.scene = v;
             j.film = r->film.film;
             j.spp = r->job.spp;

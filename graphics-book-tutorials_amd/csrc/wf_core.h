@@ -502,29 +502,40 @@ WF_HD Vec3 sphere_dir(float u1, float u2) {
     return Vec3{r * cp, r * sp, z};
 }
 
-WF_HD Ray camera_ray(const SceneView& s, int px, int py, PathKey key) {
+// Camera ray through pixel (px, py) of a width x height image seen by rig `cam`. A job of the
+// batched engines can carry its own view (rig and size) over a shared scene.
+WF_HD Ray camera_ray_view(const CameraRig& cam, int width, int height, int px, int py, PathKey key) {
     float jx = rng(key, 0, kDimPixelX), jy = rng(key, 0, kDimPixelY);
-    float u = ((float)px + jx) / (float)s.width;
-    float v = ((float)(s.height - 1 - py) + jy) / (float)s.height;
-    Vec3 target = s.camera.lower_left + s.camera.horizontal * u + s.camera.vertical * v;
+    float u = ((float)px + jx) / (float)width;
+    float v = ((float)(height - 1 - py) + jy) / (float)height;
+    Vec3 target = cam.lower_left + cam.horizontal * u + cam.vertical * v;
     Ray r;
-    r.o = s.camera.origin;
-    r.d = normalize(target - s.camera.origin);
+    r.o = cam.origin;
+    r.d = normalize(target - cam.origin);
     r.tmin = kRayEps;
     r.tmax = kInf;
     return r;
 }
 
-WF_HD PathState init_path(const SceneView& s, int px, int py, uint32_t sample, uint32_t seed) {
+WF_HD Ray camera_ray(const SceneView& s, int px, int py, PathKey key) {
+    return camera_ray_view(s.camera, s.width, s.height, px, py, key);
+}
+
+WF_HD PathState init_path_view(const CameraRig& cam, int width, int height, int px, int py, uint32_t sample,
+                               uint32_t seed) {
     PathState ps;
-    ps.key = make_path_key(seed, (uint32_t)(py * s.width + px), sample);
-    ps.ray = camera_ray(s, px, py, ps.key);
+    ps.key = make_path_key(seed, (uint32_t)(py * width + px), sample);
+    ps.ray = camera_ray_view(cam, width, height, px, py, ps.key);
     ps.throughput = Vec3{1.0f, 1.0f, 1.0f};
     ps.L = Vec3{0.0f, 0.0f, 0.0f};
     ps.depth = 0;
     ps.specular = 1;
     ps.prev_pdf = 0.0f;
     return ps;
+}
+
+WF_HD PathState init_path(const SceneView& s, int px, int py, uint32_t sample, uint32_t seed) {
+    return init_path_view(s.camera, s.width, s.height, px, py, sample, seed);
 }
 
 WF_HD Surface make_surface(const SceneView& s, const Ray& ray, const Hit& h) {

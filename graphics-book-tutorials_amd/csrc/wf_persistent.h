@@ -43,6 +43,10 @@ constexpr int kMaxJobs = 4096;
 
 struct PersistentJob {
     const SceneView* scene = nullptr;   // view with device pointers (one per distinct scene)
+    // The job's own view of the scene: image size and camera rig (has_view = false: the scene's).
+    bool has_view = false;
+    int width = 0, height = 0;
+    CameraRig cam{};
     DeviceFilm film{};
     int first_sample = 0;
     int spp = 1;
@@ -70,6 +74,7 @@ struct PersistentResult {
 struct JobDev {
     int scene;
     int width, height, max_depth;
+    CameraRig cam;
     uint32_t first_sample, spp, seed, pad;
     unsigned long long work_begin, work_end;
     float4* sbuf;

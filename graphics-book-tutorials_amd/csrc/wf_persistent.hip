@@ -87,7 +87,6 @@ __device__ __forceinline__ PathState new_path(const PCtx& c, uint32_t n_jobs, un
     uint32_t jid = 0;
     if constexpr (!SINGLE) jid = (uint32_t)find_job(c.jobs, n_jobs, w);
     const JobDev& job = job_of<SINGLE>(c, jid);
-    const SceneView& s = scene_of<SINGLE>(c, job.scene);
     const uint32_t local = (uint32_t)(w - job.work_begin);
     const uint32_t W = (uint32_t)job.width, WH = W * (uint32_t)job.height;
     const uint32_t q = local / WH;
@@ -95,7 +94,7 @@ __device__ __forceinline__ PathState new_path(const PCtx& c, uint32_t n_jobs, un
     tile_pixel(local - q * WH, W, (uint32_t)job.height, &px, &py);
     *jid_out = jid;
     *sidx_out = q * WH + py * W + px;
-    return init_path(s, (int)px, (int)py, job.first_sample + q, job.seed);
+    return init_path_view(job.cam, job.width, job.height, (int)px, (int)py, job.first_sample + q, job.seed);
 }
 
 // Stages the LDS layout (BVH tops and primitive records); the whole block calls it.
@@ -462,6 +461,10 @@ __global__ void pw_resolve_kernel(JobDev job) {
     job.film.lum2[p] = lum2;
 }
 
+uint64_t job_pixels(const PersistentJob& j) {
+    return j.has_view ? (uint64_t)j.width * (uint64_t)j.height : (uint64_t)j.scene->width * j.scene->height;
+}
+
 int grid_for(const void* fn, int block, size_t lds, int cus) {
     int per = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, block, lds) != hipSuccess || per < 1) per = 1;
@@ -683,7 +686,9 @@ int PersistentEngine::admit(const PersistentJob& job) {
     if ((int)m.hjobs.size() >= kMaxJobs) throw std::runtime_error("persistent: job table full");
     if (job.spp < 1 || job.scene == nullptr) throw std::runtime_error("persistent: invalid job");
     if (job.max_depth < 0 || job.max_depth > 0xffff) throw std::runtime_error("persistent: max_depth out of range");
-    const uint64_t WH = (uint64_t)job.scene->width * job.scene->height;
+    const int jw = job.has_view ? job.width : job.scene->width, jh = job.has_view ? job.height : job.scene->height;
+    if (jw < 1 || jh < 1) throw std::runtime_error("persistent: job image size must be positive");
+    const uint64_t WH = (uint64_t)jw * (uint64_t)jh;
     const uint64_t work = WH * (uint64_t)job.spp;
     if (work >= (1ull << 32)) throw std::runtime_error("persistent: job has 2^32 or more paths");
     // Scene table: kept across job resets; an entry matches by address and by contents (a view
@@ -707,8 +712,9 @@ int PersistentEngine::admit(const PersistentJob& job) {
     }
     JobDev d{};
     d.scene = sc;
-    d.width = job.scene->width;
-    d.height = job.scene->height;
+    d.width = jw;
+    d.height = jh;
+    d.cam = job.has_view ? job.cam : job.scene->camera;
     d.max_depth = job.max_depth;
     d.first_sample = (uint32_t)job.first_sample;
     d.spp = (uint32_t)job.spp;
@@ -907,7 +913,7 @@ void PersistentEngine::render(const PersistentJob* jobs, int n_jobs, const Persi
     uint64_t total = 0;
     for (PersistentJob& j : rest) {
         if (j.spp < 0 || j.scene == nullptr) throw std::runtime_error("persistent: invalid job");
-        total += (uint64_t)j.scene->width * j.scene->height * (uint64_t)j.spp;
+        total += job_pixels(j) * (uint64_t)j.spp;
     }
     // No more slots than paths in the batch.
     PersistentTuning tt = t;
@@ -920,7 +926,7 @@ void PersistentEngine::render(const PersistentJob* jobs, int n_jobs, const Persi
         uint64_t budget = sbuf_budget_;
         for (PersistentJob& j : rest) {
             if (j.spp <= 0) continue;
-            const uint64_t per_sample = (uint64_t)j.scene->width * j.scene->height * sizeof(float4);
+            const uint64_t per_sample = job_pixels(j) * sizeof(float4);
             uint64_t n = budget / per_sample;
             if (n == 0 && chunk.empty()) n = 1;
             if (n == 0) break;
@@ -965,7 +971,7 @@ PersistentTuning autotune_persistent(const PersistentJob* jobs, int n_jobs, hipS
     uint64_t work = 0;
     for (PersistentJob& j : probe) {
         j.spp = std::min(j.spp, probe_spp);
-        work += (uint64_t)j.scene->width * j.scene->height * j.spp;
+        work += job_pixels(j) * j.spp;
     }
     PersistentEngine eng;
     hipEvent_t a, b;

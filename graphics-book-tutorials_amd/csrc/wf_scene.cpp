@@ -755,15 +755,22 @@ Scene make_scene(const std::string& name, int width, int height) {
     return s;
 }
 
-void orbit_camera(Scene& s, float yaw_deg) {
-    const Camera& c = s.camera;
+Camera orbited(const Camera& c, float yaw_deg) {
     float dx = c.position.x - c.look_at.x, dy = c.position.y - c.look_at.y, dz = c.position.z - c.look_at.z;
     float dist = std::sqrt(dx * dx + dy * dy + dz * dz);
     float pitch = std::asin(std::max(-1.0f, std::min(1.0f, dy / std::max(dist, 1e-6f))));
     float yaw = yaw_deg * kPi / 180.0f;
     float cp = std::cos(pitch);
-    s.camera.position = Vec3{c.look_at.x + dist * cp * std::sin(yaw), c.look_at.y + dist * std::sin(pitch),
-                             c.look_at.z + dist * cp * std::cos(yaw)};
+    Camera o = c;
+    o.position = Vec3{c.look_at.x + dist * cp * std::sin(yaw), c.look_at.y + dist * std::sin(pitch),
+                      c.look_at.z + dist * cp * std::cos(yaw)};
+    return o;
+}
+
+void orbit_camera(Scene& s, float yaw_deg) { s.camera = orbited(s.camera, yaw_deg); }
+
+CameraRig view_rig(const Scene& s, int width, int height, bool orbit, float yaw_deg) {
+    return make_camera_rig(orbit ? orbited(s.camera, yaw_deg) : s.camera, width, height);
 }
 
 }  // namespace wf

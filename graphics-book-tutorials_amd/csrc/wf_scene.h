@@ -102,6 +102,11 @@ std::vector<std::string> scene_names();
 // Orbit the camera about its look-at point to an absolute yaw (degrees), keeping the distance
 // and pitch of the current position (the viewer's orbit, src/viewer/viewer.cpp:80-90).
 void orbit_camera(Scene& s, float yaw_deg);
+Camera orbited(const Camera& c, float yaw_deg);
+
+// Camera rig of a view of the scene: its own image size and, with orbit, the camera orbited to
+// yaw_deg. Jobs of the batched engines render such views of one shared scene.
+CameraRig view_rig(const Scene& s, int width, int height, bool orbit, float yaw_deg);
 
 // View over the device-layout arrays held by the scene (host pointers).
 SceneView host_view(const Scene& s);

@@ -86,7 +86,8 @@ class BounceTrace(C.Structure):
 
 class Job(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("spp", C.c_int), ("seed", C.c_uint32), ("max_depth", C.c_int),
-                ("rgb", C.POINTER(C.c_float)), ("lum_var", C.POINTER(C.c_float))]
+                ("rgb", C.POINTER(C.c_float)), ("lum_var", C.POINTER(C.c_float)), ("width", C.c_int),
+                ("height", C.c_int), ("yaw_deg", C.c_float), ("orbit", C.c_int)]
 
 
 class BatchStats(C.Structure):
@@ -108,7 +109,7 @@ _lib = None
 
 # Every symbol declared in include/wfamd.h.
 EXPORTS = [
-    "wf_last_error", "wf_version", "wf_scene_create", "wf_scene_from_arrays", "wf_scene_orbit",
+    "wf_last_error", "wf_version", "wf_scene_create", "wf_scene_from_arrays", "wf_scene_orbit", "wf_scene_view",
     "wf_scene_destroy", "wf_scene_info_get", "wf_scene_export", "wf_scene_bvh", "wf_scene_camera_rig",
     "wf_renderer_create", "wf_renderer_destroy", "wf_renderer_render", "wf_renderer_reset",
     "wf_renderer_run_passes", "wf_renderer_download", "wf_renderer_download_mean", "wf_renderer_download_sums",
@@ -136,6 +137,7 @@ def lib():
         "wf_scene_create": (I, [C.c_char_p, I, I, C.POINTER(P)]),
         "wf_scene_from_arrays": (I, [C.POINTER(SceneDesc), C.POINTER(P)]),
         "wf_scene_orbit": (I, [P, C.c_float]), "wf_scene_destroy": (None, [P]),
+        "wf_scene_view": (I, [P, I, I, C.c_float, I, C.POINTER(P)]),
         "wf_scene_info_get": (I, [P, C.POINTER(SceneInfo)]),
         "wf_scene_export": (I, [P, F, F, C.POINTER(C.c_int), F, F, F]),
         "wf_scene_bvh": (I, [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -247,6 +249,14 @@ class Scene:
 
     def orbit(self, yaw_deg):
         _check(lib().wf_scene_orbit(self._h, float(yaw_deg)))
+
+    def view(self, width, height, yaw_deg=None):
+        """A copy of this scene seen at width x height and, if yaw_deg is given, from the orbited
+        camera (the standalone scene of a job view)."""
+        h = C.c_void_p()
+        _check(lib().wf_scene_view(self._h, int(width), int(height), float(yaw_deg or 0.0),
+                                   0 if yaw_deg is None else 1, C.byref(h)))
+        return Scene(handle=h)
 
     def arrays(self):
         i = self.info()
@@ -410,16 +420,25 @@ def test_trace_paths(scene, pxys, seed=1234, max_depth=8, device=False):
     return L, rays
 
 
+def make_job(scene, spp, seed=1234, max_depth=8, width=0, height=0, yaw_deg=None):
+    """A Job and its output arrays. width/height 0: the scene's size; yaw_deg: orbit the camera."""
+    i = scene.info()
+    w, h = (width or i.width), (height or i.height)
+    rgb = np.zeros(3 * w * h, np.float32)
+    var = np.zeros(w * h, np.float32)
+    job = Job(scene.handle, spp, seed, max_depth, _fptr(rgb), _fptr(var), int(width), int(height),
+              float(yaw_deg or 0.0), 0 if yaw_deg is None else 1)
+    return job, (rgb, var)
+
+
 def make_jobs(specs):
-    """specs: list of (scene, spp, seed, max_depth). Returns (ctypes Job array, output arrays)."""
+    """specs: list of (scene, spp, seed, max_depth[, width, height[, yaw_deg]]). Returns (ctypes Job
+    array, output arrays)."""
     arr = (Job * len(specs))()
     outs = []
-    for k, (scene, spp, seed, depth) in enumerate(specs):
-        i = scene.info()
-        rgb = np.zeros(3 * i.width * i.height, np.float32)
-        var = np.zeros(i.width * i.height, np.float32)
-        outs.append((rgb, var))
-        arr[k] = Job(scene.handle, spp, seed, depth, _fptr(rgb), _fptr(var))
+    for k, spec in enumerate(specs):
+        arr[k], out = make_job(*spec)
+        outs.append(out)
     return arr, outs
 
 
@@ -456,11 +475,8 @@ class Server:
         if _lib is not None:
             self.close()
 
-    def submit(self, scene, spp, seed=1234, max_depth=8):
-        i = scene.info()
-        rgb = np.zeros(3 * i.width * i.height, np.float32)
-        var = np.zeros(i.width * i.height, np.float32)
-        job = Job(scene.handle, spp, seed, max_depth, _fptr(rgb), _fptr(var))
+    def submit(self, scene, spp, seed=1234, max_depth=8, width=0, height=0, yaw_deg=None):
+        job, (rgb, var) = make_job(scene, spp, seed, max_depth, width, height, yaw_deg)
         rid = C.c_int64()
         _check(lib().wf_server_submit(self._h, C.byref(job), C.byref(rid)))
         self._keep[rid.value] = (scene, rgb, var)

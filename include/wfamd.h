@@ -126,6 +126,10 @@ const char* wf_version(void);
 int wf_scene_create(const char* name, int width, int height, wf_scene** out);
 int wf_scene_from_arrays(const wf_scene_desc* desc, wf_scene** out);
 int wf_scene_orbit(wf_scene* scene, float yaw_deg);  /* viewer orbit, src/viewer/viewer.cpp:86-90 */
+/* A copy of `scene` seen through another view: image size width x height and, with orbit = 1, the
+ * camera orbited to yaw_deg. Geometry and BVH are copied, not rebuilt. This is the standalone
+ * scene of one wf_job view (the reference builds a new scene per camera, src/viewer/viewer.cpp:114-120). */
+int wf_scene_view(const wf_scene* scene, int width, int height, float yaw_deg, int orbit, wf_scene** out);
 void wf_scene_destroy(wf_scene* scene);
 int wf_scene_info_get(const wf_scene* scene, wf_scene_info* out);
 /* Exports (counts in elements; pass NULL to skip an array). nodes: reference BvhNode layout,
@@ -206,6 +210,12 @@ typedef struct wf_job {
     int max_depth;
     float* rgb;       /* 3*W*H mean, may be NULL */
     float* lum_var;   /* W*H, may be NULL */
+    /* The job's view of the scene (many jobs can share one scene and its BVH):
+     * width, height: image size, 0 = the scene's; orbit = 1: the camera orbited to yaw_deg
+     * (degrees) about its look-at point (as wf_scene_orbit), 0: the scene's camera. */
+    int width, height;
+    float yaw_deg;
+    int orbit;
 } wf_job;
 
 typedef struct wf_batch_stats {

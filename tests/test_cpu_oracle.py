@@ -56,6 +56,29 @@ def test_invalid_inputs_raise(wf):
             [0, 0, 5, 0, 0, 0, 0, 1, 0, 40], np.float32), 8, 8)
 
 
+def test_scene_view_equals_a_fresh_orbited_scene(wf):
+    # A job view (Scene.view: copied geometry, own size and orbited camera) is the scene that
+    # make_scene + orbit would build, without rebuilding the BVH: same rig, same BVH, same image.
+    base = wf.Scene("cornell555", 64, 36)
+    for w, h, yaw in ((40, 30, 75.0), (23, 31, None), (64, 36, 359.5)):
+        v = base.view(w, h, yaw)
+        fresh = wf.Scene("cornell555", w, h)
+        if yaw is not None:
+            fresh.orbit(yaw)
+        assert v.info().width == w and v.info().height == h
+        assert np.array_equal(v.camera_rig().view(np.uint32), fresh.camera_rig().view(np.uint32))
+        for a, b in zip(v.bvh(), fresh.bvh()):
+            assert np.array_equal(a, b)
+        p = wf.params(spp=2, threads=4)
+        (a, _, _), _ = wf.render_cpu_sums(v, 0, 2, p)
+        (b, _, _), _ = wf.render_cpu_sums(fresh, 0, 2, p)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    with pytest.raises(wf.InvalidArgument):
+        base.view(0, 10)
+    with pytest.raises(wf.InvalidArgument):
+        base.view(10, 9000)
+
+
 # ---------------------------------------------------------------------------------------------
 # RNG and trigonometry
 # ---------------------------------------------------------------------------------------------

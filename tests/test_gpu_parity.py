@@ -343,6 +343,51 @@ def test_serving_results_equal_alone_and_in_mix(wf, scenes):
     srv.close()
 
 
+def test_batch_job_views_equal_standalone_view_renders(wf, scenes):
+    # Jobs with their own image size and orbit camera over shared scenes (one BVH per scene): each
+    # equals a standalone render of the same view (Scene.view copies geometry, sets the camera).
+    base = {n: scenes(n, 64, 36) for n in ("cornell555", "sphere_grid", "heightfield")}
+    views = [("sphere_grid", 40, 30, 30.0, 2), ("sphere_grid", 64, 36, 200.0, 1), ("heightfield", 33, 17, 95.5, 2),
+             ("cornell555", 24, 40, None, 3), ("heightfield", 50, 20, 359.0, 1), ("sphere_grid", 17, 9, None, 4)]
+    specs = [(base[n], spp, 7 + k, 8, w, h, yaw) for k, (n, w, h, yaw, spp) in enumerate(views)]
+    outs, st = wf.batch_render(specs, wf.params(pool_paths=2048))
+    assert st.queue_overflows == 0 and st.invalid_hits == 0
+    total = 0
+    for (n, w, h, yaw, spp), spec, (rgb, var) in zip(views, specs, outs):
+        v = base[n].view(w, h, yaw)
+        r = wf.Renderer(v, "megakernel")
+        want, wvar, wst = r.render(wf.params(spp=spp, seed=spec[2]))
+        r.close()
+        assert rgb.size == 3 * w * h
+        assert_bitwise(rgb, want, f"view {n} {w}x{h} yaw {yaw} rgb")
+        assert_bitwise(var, wvar, f"view {n} {w}x{h} yaw {yaw} var")
+        total += wst.extension_rays + wst.shadow_rays
+    assert st.extension_rays + st.shadow_rays == total
+
+
+def test_serving_job_views_equal_standalone(wf, scenes):
+    base = scenes("sphere_grid", 64, 36)
+    srv = wf.Server(budget_bytes=8 << 30, tuning=wf.params(pool_paths=4096))
+    reqs = [(48, 27, 10.0, 3), (80, 45, 123.0, 2), (31, 29, None, 4)]
+    ids = []
+    for k, (w, h, yaw, spp) in enumerate(reqs):
+        ids.append(srv.submit(base, spp, 40 + k, 8, w, h, yaw))
+        srv.step(1)
+    for _ in range(2000):
+        if all(srv.poll(i) == 1 for i in ids):
+            break
+        srv.step(1)
+    assert all(srv.poll(i) == 1 for i in ids)
+    for k, (rid, (w, h, yaw, spp)) in enumerate(zip(ids, reqs)):
+        rgb, var = srv.result(rid)
+        r = wf.Renderer(base.view(w, h, yaw), "megakernel")
+        want, wvar, _ = r.render(wf.params(spp=spp, seed=40 + k))
+        r.close()
+        assert_bitwise(rgb, want, "served view rgb")
+        assert_bitwise(var, wvar, "served view var")
+    srv.close()
+
+
 def test_serving_budget_rejects_and_defers(wf, scenes):
     s = scenes("sphere_grid", 64, 36)
     probe = wf.Server(budget_bytes=1 << 40, tuning=wf.params(pool_paths=4096))
