@@ -20,7 +20,7 @@ constexpr int kDefaultPool = 1 << 22;
 constexpr int kDefaultBlock = 512;
 constexpr int kDefaultGraphIters = 16;
 constexpr int kDefaultLdsKiB = 64;
-constexpr int kDefaultRefill = 16;
+constexpr int kDefaultRefill = 32;
 constexpr int kDefaultLeafBatch = 16;
 constexpr int kDefaultDirBins = 1;
 constexpr uint32_t kIdle = 0xFFFFFFFFu;
@@ -134,26 +134,43 @@ __device__ __forceinline__ void count_done(uint32_t* job_done, bool pred, uint32
     }
 }
 
-// Items of a traversal kernel: every wave of the grid owns one contiguous range of n.
-struct WaveRange {
-    uint32_t begin, end, per;
+// Rays of a traversal kernel are dealt to the waves in slices of kFeedSlice consecutive items,
+// round-robin: wave w takes slices w, w + W, w + 2W, ... of the W waves of the grid. Each slice
+// is spatially coherent (consecutive items are neighbouring pixels or their survivors) while every
+// wave samples the whole image, so the grid stays balanced however the cost of a ray varies across
+// the image, without atomics. Wave-uniform state.
+constexpr uint32_t kFeedSlice = 128;
+
+struct WaveFeed {
+    uint32_t cursor = 0, end = 0;   // rest of the current slice [cursor, end)
+    uint32_t slice = 0, stride = 0;
+    bool dry = false;               // no slice left
 };
 
-__device__ __forceinline__ WaveRange wave_range(uint32_t n) {
+__device__ __forceinline__ void feed_first(WaveFeed& f, uint32_t n) {
     const uint32_t wpb = blockDim.x >> 6;
-    const uint32_t nw = gridDim.x * wpb;
-    const uint32_t per = (n + nw - 1) / nw;
-    const uint32_t gw = blockIdx.x * wpb + (threadIdx.x >> 6);
-    WaveRange r;
-    r.per = per;
-    r.begin = min(gw * per, n);
-    r.end = min(r.begin + per, n);
-    return r;
+    f.stride = gridDim.x * wpb;
+    f.slice = blockIdx.x * wpb + (threadIdx.x >> 6);
+    f.cursor = min(f.slice * kFeedSlice, n);
+    f.end = min(f.cursor + kFeedSlice, n);
+    f.dry = f.cursor >= n;
 }
 
-// True when no wave of this block owns an item (the block returns before staging LDS).
-__device__ __forceinline__ bool block_idle(uint32_t n, uint32_t per) {
-    return blockIdx.x * (blockDim.x >> 6) * per >= n;
+__device__ __forceinline__ void feed_next(WaveFeed& f, uint32_t n) {
+    f.slice += f.stride;
+    const unsigned long long c = (unsigned long long)f.slice * kFeedSlice;
+    if (c >= n) {
+        f.dry = true;
+        f.cursor = f.end = n;
+    } else {
+        f.cursor = (uint32_t)c;
+        f.end = min(f.cursor + kFeedSlice, n);
+    }
+}
+
+// True when no wave of this block has a slice (the block returns before staging LDS).
+__device__ __forceinline__ bool block_idle(uint32_t n) {
+    return (unsigned long long)blockIdx.x * (blockDim.x >> 6) * kFeedSlice >= n;
 }
 
 __global__ void pw_init_kernel(PState* st) {
@@ -200,11 +217,10 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
     const PState* st = c.st;
     const uint32_t n_act = st->it_act, n_live = st->it_live, n_jobs = st->n_jobs;
     const unsigned long long wbase = st->it_work_base;
-    const WaveRange wr = wave_range(n_act);
-    if (block_idle(n_act, wr.per)) return;
+    if (block_idle(n_act)) return;
     stage_lds<SINGLE>(c, lds, &sp);
-    uint32_t cursor = wr.begin;
-    const uint32_t end = wr.end;
+    WaveFeed f;
+    feed_first(f, n_act);
     LdsNodes src{};
     int nn = 0, sc = 0;
     if constexpr (SINGLE) {
@@ -218,9 +234,10 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
     int leaf = -1;
     for (;;) {
         const uint64_t idle = __ballot(item == kIdle);
-        if (idle != 0 && cursor < end) {
-            const uint32_t cand = cursor + mbcnt64(idle);
-            if (item == kIdle && cand < end) {
+        if (idle != 0 && f.cursor >= f.end && !f.dry) feed_next(f, n_act);
+        if (idle != 0 && f.cursor < f.end) {
+            const uint32_t cand = f.cursor + mbcnt64(idle);
+            if (item == kIdle && cand < f.end) {
                 item = cand;
                 Ray r;
                 if (item < n_live) {
@@ -242,10 +259,13 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
                 tr = traversal_begin(r);
                 if (nn <= 0) tr.node = -1;
             }
-            cursor = min(cursor + (uint32_t)__popcll(idle), end);
+            f.cursor = min(f.cursor + (uint32_t)__popcll(idle), f.end);
         }
-        if (__ballot(item != kIdle) == 0) break;
-        wave_traverse(nn, src, tr, leaf, false, cursor < end ? c.refill : 1, c.leaf_batch);
+        if (__ballot(item != kIdle) == 0) {
+            if (f.dry) break;
+            continue;   // slice used up: take the next one
+        }
+        wave_traverse(nn, src, tr, leaf, false, (f.dry && f.cursor >= f.end) ? 1 : c.refill, c.leaf_batch);
         if (item != kIdle && tr.node < 0 && leaf < 0) {
             Hit h{tr.r.tmax, tr.prim};
             if (!hit_is_valid(scene_of<SINGLE>(c, sc), h)) {
@@ -379,11 +399,10 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
     extern __shared__ Node lds[];
     __shared__ LdsPlan sp;
     const uint32_t n = c.st->n_shadow;
-    const WaveRange wr = wave_range(n);
-    if (block_idle(n, wr.per)) return;
+    if (block_idle(n)) return;
     stage_lds<SINGLE>(c, lds, &sp);
-    uint32_t cursor = wr.begin;
-    const uint32_t end = wr.end;
+    WaveFeed f;
+    feed_first(f, n);
     LdsNodes src{};
     int nn = 0;
     if constexpr (SINGLE) {
@@ -397,9 +416,10 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
     int leaf = -1;
     for (;;) {
         const uint64_t idle = __ballot(item == kIdle);
-        if (idle != 0 && cursor < end) {
-            const uint32_t cand = cursor + mbcnt64(idle);
-            if (item == kIdle && cand < end) {
+        if (idle != 0 && f.cursor >= f.end && !f.dry) feed_next(f, n);
+        if (idle != 0 && f.cursor < f.end) {
+            const uint32_t cand = f.cursor + mbcnt64(idle);
+            if (item == kIdle && cand < f.end) {
                 item = cand;
                 const float4 o = c.SQ.o[item], d = c.SQ.d[item];
                 tag = (uint32_t)__float_as_int(d.w);
@@ -416,10 +436,13 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
                 tr = traversal_begin(r);
                 if (nn <= 0) tr.node = -1;
             }
-            cursor = min(cursor + (uint32_t)__popcll(idle), end);
+            f.cursor = min(f.cursor + (uint32_t)__popcll(idle), f.end);
         }
-        if (__ballot(item != kIdle) == 0) break;
-        wave_traverse(nn, src, tr, leaf, true, cursor < end ? c.refill : 1, c.leaf_batch);
+        if (__ballot(item != kIdle) == 0) {
+            if (f.dry) break;
+            continue;   // slice used up: take the next one
+        }
+        wave_traverse(nn, src, tr, leaf, true, (f.dry && f.cursor >= f.end) ? 1 : c.refill, c.leaf_batch);
         bool fin = false;
         uint32_t jid = 0;
         if (item != kIdle && tr.node < 0 && leaf < 0) {
@@ -477,6 +500,7 @@ struct PersistentEngine::Impl {
     hipStream_t stream = nullptr;
     DevCounters* ctr = nullptr;
     int block = kDefaultBlock, graph_iters = kDefaultGraphIters, lds_bytes = kDefaultLdsKiB * 1024, cus = 0;
+    int lds_max = 0;   // LDS a workgroup may use on this device (bytes)
     int refill = kDefaultRefill;
     int leaf_batch = kDefaultLeafBatch;
     int dir_bins = kDefaultDirBins;
@@ -564,6 +588,8 @@ void PersistentEngine::ensure_pool(const PersistentTuning& t) {
         int dev = 0;
         check(hipGetDevice(&dev), "hipGetDevice");
         check(hipDeviceGetAttribute(&m.cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+        check(hipDeviceGetAttribute(&m.lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev),
+              "hipDeviceGetAttribute");
     }
     int cap = t.pool_paths > 0 ? t.pool_paths : kDefaultPool;
     cap = (cap + 1023) & ~1023;
@@ -581,7 +607,9 @@ void PersistentEngine::ensure_pool(const PersistentTuning& t) {
     m.ctx.dir_bins = m.dir_bins;
     m.graph_iters = t.graph_iters > 0 ? ((t.graph_iters + 1) & ~1) : kDefaultGraphIters;
     const int kib = t.lds_kib > 0 ? t.lds_kib : kDefaultLdsKiB;
-    if (kib > 64) throw std::runtime_error("persistent: at most 64 KiB of LDS per workgroup");
+    // The kernels' own LDS (the plan and the shade round counters) stays below 1 KiB.
+    if (kib * 1024 > m.lds_max - 1024)
+        throw std::runtime_error("persistent: LDS budget above what a workgroup may use on this device");
     if (m.lds_bytes != kib * 1024) {
         m.lds_bytes = kib * 1024;
         if (!m.hscenes.empty()) {
@@ -655,20 +683,33 @@ void PersistentEngine::reset_jobs() {
     check(hipGetLastError(), "persistent init");
 }
 
-// Splits the LDS budget evenly over the scenes: BVH nodes first (the breadth-first top), then the
-// scene's primitive records if all of them fit in what is left of its share.
+// Splits the LDS budget over the scenes, smallest need first: each scene gets what it needs (all
+// of its nodes, then its primitive records) up to an even share of what is left, so small scenes
+// are served whole from LDS and large ones share the rest (their breadth-first tops).
 void PersistentEngine::update_lds_plan() {
     Impl& m = *m_;
     LdsPlan p{};
     p.n = (int)m.hscenes.size();
     if (p.n > 0) {
-        const int share = m.lds_bytes / p.n;
+        auto need = [&](int k) {
+            const SceneView& s = m.hscenes[k];
+            return (long long)s.n_nodes * (long long)sizeof(Node) + (long long)s.n_prims * 16;
+        };
+        std::vector<int> order(p.n);
+        for (int k = 0; k < p.n; ++k) order[k] = k;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return need(a) < need(b); });
+        long long left = m.lds_bytes;
+        for (int i = 0; i < p.n; ++i) {
+            const int k = order[i];
+            const SceneView& s = m.hscenes[k];
+            const long long share = left / (p.n - i);
+            p.node_cnt[k] = (int)std::min<long long>(s.n_nodes, share / (long long)sizeof(Node));
+            const long long rest = share - (long long)p.node_cnt[k] * (long long)sizeof(Node);
+            p.prim_cnt[k] = (p.node_cnt[k] == s.n_nodes && (long long)s.n_prims * 16 <= rest) ? s.n_prims : 0;
+            left -= (long long)p.node_cnt[k] * (long long)sizeof(Node) + (long long)p.prim_cnt[k] * 16;
+        }
         int nodes = 0, prims = 0;
         for (int k = 0; k < p.n; ++k) {
-            const SceneView& s = m.hscenes[k];
-            p.node_cnt[k] = std::min(s.n_nodes, share / (int)sizeof(Node));
-            const int left = share - p.node_cnt[k] * (int)sizeof(Node);
-            p.prim_cnt[k] = (p.node_cnt[k] == s.n_nodes && s.n_prims * 16 <= left) ? s.n_prims : 0;
             p.node_off[k] = nodes;
             p.prim_off[k] = prims;
             nodes += p.node_cnt[k];
@@ -754,6 +795,10 @@ PersistentEngine::LaunchPlan PersistentEngine::make_plan(bool single) {
     const void* fx = single ? (const void*)pw_extend_kernel<true> : (const void*)pw_extend_kernel<false>;
     const void* fs = single ? (const void*)pw_shade_kernel<true> : (const void*)pw_shade_kernel<false>;
     const void* fw = single ? (const void*)pw_shadow_kernel<true> : (const void*)pw_shadow_kernel<false>;
+    if (L.lds_bytes > 64 * 1024)
+        for (const void* fn : {fx, fw})
+            check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.lds_bytes),
+                  "hipFuncSetAttribute");
     L.gx = grid_for(fx, m.block, L.lds_bytes, m.cus);
     L.gs = grid_for(fs, m.block, 0, m.cus);
     L.gw = grid_for(fw, m.block, L.lds_bytes, m.cus);
@@ -1032,7 +1077,7 @@ PersistentTuning autotune_persistent(const PersistentJob* jobs, int n_jobs, hipS
     base = best;
     for (int p : pools) {
         if (p > top) continue;
-        for (int kib : {32, 64}) {
+        for (int kib : {32, 64, 128}) {
             if (p == base.pool_paths && kib == base.lds_kib) continue;
             PersistentTuning t = base;
             t.pool_paths = p;

@@ -263,6 +263,23 @@ void WavefrontEngine::run_pass(const DeviceFilm& film, DevCounters* ctr, uint32_
     wf_init_kernel<<<blocks_for(P), kThreads, 0, stream>>>(scene_, pool_, active_[0], sample, seed);
     check(hipGetLastError(), "wavefront init");
     mark(&StageTimes::other);
+    // Algorithmic device-memory bytes per item of each stage (PathPool / ShadowQueue layouts):
+    //   init      76 per pixel: ray, throughput, radiance (64) + key 8 + active index 4
+    //   extend    44 per active path: index 4, ray 32, hit 8
+    //   classify  104 per active path (index 4, hit 8, state 72, radiance 16, alive flag 4)
+    //             + 36 per path queued for shading (surface 32, queue entry 4)
+    //   shade     144 per queue entry (entry 4, surface 32, state 56 in, 48 out, alive flag 4)
+    //             + 48 per shadow ray pushed (Lambert)
+    //   shadow    48 per shadow ray (origin, direction, contribution); the radiance
+    //             read-modify-write of unoccluded rays is not counted
+    //   compact   24 per active path (mark: index 4, alive 4, flag 4; reduce: flag 4; scatter:
+    //             flag 4, index 4) + 4 per surviving path
+    //   finish    56 per pixel: radiance 16, film read-modify-write 40
+    if (timing) {
+        stages->items[7] += P;
+        stages->launches[7] += 1;
+        stages->bytes[7] += 76LL * P;
+    }
 
     uint32_t n_active = P;
     int cur = 0, bounce = 0;
@@ -306,12 +323,16 @@ void WavefrontEngine::run_pass(const DeviceFilm& film, DevCounters* ctr, uint32_
         if (timing) {
             uint32_t q[4];
             check(hipMemcpy(q, counters_, sizeof(q), hipMemcpyDeviceToHost), "copy queue counts");
-            const long long items[8] = {n_active, n_active, std::min(q[0], cap), std::min(q[1], cap),
-                                        std::min(q[2], cap), std::min(q[3], cap), n_active, 0};
-            const long long launches[8] = {1, 1, 1, 1, 1, 1, 5, 2};
+            const long long qL = std::min(q[0], cap), qM = std::min(q[1], cap), qD = std::min(q[2], cap),
+                            qS = std::min(q[3], cap), na = n_active;
+            const long long items[8] = {na, na, qL, qM, qD, qS, na, 0};
+            const long long launches[8] = {2, 1, 1, 1, 1, 1, 5, 0};
+            const long long bytes[8] = {44 * na, 104 * na + 36 * (qL + qM + qD), 144 * qL + 48 * qS, 144 * qM,
+                                        144 * qD, 48 * qS, 24 * na + 4LL * next, 0};
             for (int k = 0; k < 8; ++k) {
                 stages->items[k] += items[k];
                 stages->launches[k] += launches[k];
+                stages->bytes[k] += bytes[k];
             }
         }
         if (trace != nullptr) {
@@ -343,6 +364,11 @@ void WavefrontEngine::run_pass(const DeviceFilm& film, DevCounters* ctr, uint32_
     wf_finish_kernel<<<blocks_for(P), kThreads, 0, stream>>>(film, pool_, pixels_);
     check(hipGetLastError(), "wavefront finish");
     mark(&StageTimes::other);
+    if (timing) {
+        stages->items[7] += P;
+        stages->launches[7] += 1;
+        stages->bytes[7] += 56LL * P;
+    }
 }
 
 }  // namespace wf::gpu

@@ -159,6 +159,24 @@ def run_condition(scene, mode, params, reps, warmup, w, h):
     return c, first, extra
 
 
+def stage_profile(scene, mode, spp, seed, depth, tuned=None):
+    """One stage-timed render (HIP events around every stage, synchronising): per stage ms,
+    items, launches, algorithmic device-memory bytes and the GB/s they imply (the reference's
+    stage_timing run, src/bench.cpp:166-172)."""
+    kw = {}
+    if tuned is not None:
+        kw = {k: getattr(tuned, k) for k in ("block_size", "pool_paths", "lds_kib", "refill", "leaf_batch", "dir_bins")}
+    r = wfamd.Renderer(scene, mode)
+    _, _, st = r.render(wfamd.params(spp=spp, seed=seed, max_depth=depth, stage_timing=True, **kw))
+    r.close()
+    out = {}
+    for k, v in st.stages().items():
+        v = dict(v)
+        v["gbs"] = v["bytes"] / (v["ms"] / 1e3) / 1e9 if v["ms"] > 0 and v["bytes"] else None
+        out[k] = v
+    return out
+
+
 def bench_scene(cfg_id, cfg, quick):
     w, h, spp, reps, warmup = cfg["width"], cfg["height"], cfg["spp"], cfg["reps"], cfg["warmup"]
     if quick:
@@ -200,6 +218,8 @@ def bench_scene(cfg_id, cfg, quick):
                             "value": within, "threshold": PROTOCOL["se_pixel_fraction"],
                             "detail": f"{cpu_spp} spp, pixels within 3 SE; bitwise-equal pixels {same:.6f}"})
     prel, _, psame = image_diff(mk_img[0], mk_img[1], spp, pe_img[0], pe_img[1], spp)
+    stages = {"wavefront": stage_profile(scene, "wavefront", spp, seed, depth),
+              "persistent": stage_profile(scene, "persistent", spp, seed, depth, tuned)}
     doc = {"schema_version": PROTOCOL["schema_version"], "scene_name": cfg["scene"],
            "scene_hash": f"{scene.info().hash:016x}",
            "width": w, "height": h, "spp": spp, "max_depth": depth, "reps": reps, "warmup": warmup, "seed": seed,
@@ -214,6 +234,7 @@ def bench_scene(cfg_id, cfg, quick):
         "tuning": {k: getattr(tuned, k) for k in ("block_size", "pool_paths", "lds_kib", "refill", "leaf_batch",
                                                    "dir_bins")},
         "cpu_arm": cpu_x,
+        "stages": stages,
     }
     return doc
 
