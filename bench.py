@@ -24,6 +24,7 @@ sys.path.insert(0, os.path.join(ROOT, "graphics-book-tutorials_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 METRIC = "Mrays/s at 1280x720x64spp depth 8 per scene; p50/p99 render latency"
+TUNING_KEYS = ("block_size", "pool_paths", "lds_kib", "refill", "leaf_batch", "dir_bins", "shade_block")
 HBM_PEAK_GBS = 8000.0
 # Device-memory bytes per launch with work measured by rocprofv3 FETCH_SIZE and WRITE_SIZE on this
 # workload (sphere_grid 1280x720x64, default tuning), each counter scaled by what it reads on a plain
@@ -120,10 +121,9 @@ def main():
     if not args.no_autotune:
         t0 = time.perf_counter()
         tuned = wfamd.autotune([(scene, spp, seed, depth)], probe_spp=16)
-        tuning.block_size, tuning.pool_paths, tuning.lds_kib = tuned.block_size, tuned.pool_paths, tuned.lds_kib
-        tuning.refill, tuning.leaf_batch, tuning.dir_bins = tuned.refill, tuned.leaf_batch, tuned.dir_bins
-        print(f"[bench] autotune: block {tuned.block_size} pool {tuned.pool_paths} lds {tuned.lds_kib} KiB "
-              f"refill {tuned.refill} leaf batch {tuned.leaf_batch} dir bins {tuned.dir_bins} "
+        for k in TUNING_KEYS:
+            setattr(tuning, k, getattr(tuned, k))
+        print(f"[bench] autotune: {', '.join(f'{k} {getattr(tuned, k)}' for k in TUNING_KEYS)} "
               f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
 
     r = wfamd.Renderer(scene, "persistent")
@@ -172,9 +172,8 @@ def main():
     # per-kernel average launch durations, items and algorithmic device-memory bytes per launch
     # (computed by the engine from each iteration's counts; layouts in wf_persistent.hip).
     roof = None
-    prof = wfamd.params(spp=spp, seed=seed, max_depth=depth, stage_timing=True, block_size=tuning.block_size,
-                        pool_paths=tuning.pool_paths, lds_kib=tuning.lds_kib, refill=tuning.refill,
-                        leaf_batch=tuning.leaf_batch, dir_bins=tuning.dir_bins)
+    prof = wfamd.params(spp=spp, seed=seed, max_depth=depth, stage_timing=True,
+                        **{k: getattr(tuning, k) for k in TUNING_KEYS})
     r.reset()
     pst = r.run_passes(0, spp, prof)
     names = {"extend": "extend", "shade_lambert": "shade", "shadow": "shadow"}
@@ -217,9 +216,7 @@ def main():
         "data": "synthetic: procedural sphere_grid scene (1024 jittered spheres, seeded), no external data",
         "config": {"workload": f"{args.scene} {args.width}x{args.height} {spp}spp depth {depth} seed {seed} "
                                f"(material-divergent scene)", "arm": "persistent batched wavefront",
-                   "block_size": tuning.block_size, "pool_paths": tuning.pool_paths, "lds_kib": tuning.lds_kib,
-                   "refill": tuning.refill, "leaf_batch": tuning.leaf_batch, "dir_bins": tuning.dir_bins,
-                   "parallelism": "single GPU"},
+                   **{k: getattr(tuning, k) for k in TUNING_KEYS}, "parallelism": "single GPU"},
         "p50_ms": percentile(lat, 0.5), "p99_ms": percentile(lat, 0.99),
         "extension_mrays_per_s": ext / elapsed / 1e6, "rays_per_render": rays_per_render,
         "download_ms": download_ms,

@@ -66,6 +66,7 @@ wf::RenderParams to_params(const wf_render_params* p) {
     r.refill = p->refill;
     r.leaf_batch = p->leaf_batch;
     r.dir_bins = p->dir_bins;
+    r.shade_block = p->shade_block;
     return r;
 }
 
@@ -638,6 +639,7 @@ wf::gpu::PersistentTuning to_tuning(const wf_render_params* p) {
         t.refill = p->refill;
         t.leaf_batch = p->leaf_batch;
         t.dir_bins = p->dir_bins;
+        t.shade_block = p->shade_block;
     }
     return t;
 }
@@ -717,6 +719,7 @@ extern "C" int wf_batch_render(const wf_job* jobs, int n_jobs, const wf_render_p
             stats->refill = t.refill;
             stats->leaf_batch = t.leaf_batch;
             stats->dir_bins = t.dir_bins;
+            stats->shade_block = t.shade_block;
             stats->device_bytes = eng.bytes() + b.scenes.bytes();
         }
         return (int)WF_OK;
@@ -735,6 +738,7 @@ extern "C" int wf_autotune(const wf_job* jobs, int n_jobs, int probe_spp, wf_ren
         tuning->refill = best.refill;
         tuning->leaf_batch = best.leaf_batch;
         tuning->dir_bins = best.dir_bins;
+        tuning->shade_block = best.shade_block;
         return (int)WF_OK;
     });
 }

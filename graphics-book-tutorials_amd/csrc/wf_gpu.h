@@ -233,6 +233,204 @@ __device__ __forceinline__ void wave_traverse(int n_nodes, const Src& src, Trave
     }
 }
 
+// Rays of a traversal kernel are dealt to the waves in slices of `size` consecutive items,
+// round-robin: wave w takes slices w, w + W, w + 2W, ... of the W waves of the grid. Each slice
+// is spatially coherent (consecutive items are neighbouring pixels or their survivors) while every
+// wave samples the whole image, so the grid stays balanced however the cost of a ray varies across
+// the image, without atomics. Large launches (the persistent engine) use 128-ray slices; launches
+// of a few hundred thousand rays (the plain wavefront's bounces) use 64, one ray per lane, so that
+// they still occupy every SIMD. Wave-uniform state.
+constexpr uint32_t kFeedSlice = 128;
+
+struct WaveFeed {
+    uint32_t cursor = 0, end = 0;   // rest of the current slice [cursor, end)
+    uint32_t slice = 0, stride = 0, size = kFeedSlice;
+    bool dry = false;               // no slice left
+};
+
+__device__ __forceinline__ void feed_first(WaveFeed& f, uint32_t n, uint32_t size = kFeedSlice) {
+    const uint32_t wpb = blockDim.x >> 6;
+    f.size = size;
+    f.stride = gridDim.x * wpb;
+    f.slice = blockIdx.x * wpb + (threadIdx.x >> 6);
+    const unsigned long long c = (unsigned long long)f.slice * size;
+    f.cursor = c < n ? (uint32_t)c : n;
+    f.end = min(f.cursor + size, n);
+    f.dry = f.cursor >= n;
+}
+
+__device__ __forceinline__ void feed_next(WaveFeed& f, uint32_t n) {
+    f.slice += f.stride;
+    const unsigned long long c = (unsigned long long)f.slice * f.size;
+    if (c >= n) {
+        f.dry = true;
+        f.cursor = f.end = n;
+    } else {
+        f.cursor = (uint32_t)c;
+        f.end = min(f.cursor + f.size, n);
+    }
+}
+
+// True when no wave of this block has a slice (the block returns before staging LDS).
+__device__ __forceinline__ bool block_idle(uint32_t n, uint32_t size = kFeedSlice) {
+    return (unsigned long long)blockIdx.x * (blockDim.x >> 6) * size >= n;
+}
+
+// Wave64 queue push to K queues at once: the lane's value goes to queue q (q < 0: none). The first
+// active lane issues the K atomics back to back (one round trip for all queues instead of one per
+// queue); slots at or past `capacity` are dropped and counted in *overflow. Called by every active
+// lane with the same control flow. Returns the slot or ~0u.
+template <int K>
+__device__ __forceinline__ uint32_t wave_push_multi(uint32_t* const* items, uint32_t* counters, uint32_t capacity,
+                                                    uint32_t value, int q, uint32_t* overflow) {
+    uint64_t m[K];
+    uint32_t base[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        m[k] = __ballot(q == k);
+        base[k] = 0u;
+    }
+    const int leader = __ffsll((unsigned long long)__ballot(true)) - 1;
+    if ((int)lane_id() == leader) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) base[k] = m[k] ? atomicAdd(&counters[k], (uint32_t)__popcll(m[k])) : 0u;
+        uint32_t dropped = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t n = (uint32_t)__popcll(m[k]);
+            const uint32_t keep = base[k] >= capacity ? 0u : (capacity - base[k] < n ? capacity - base[k] : n);
+            dropped += n - keep;
+        }
+        if (dropped) atomicAdd(overflow, dropped);
+    }
+    uint32_t slot = 0xFFFFFFFFu;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint32_t b = (uint32_t)__shfl((int)base[k], leader);
+        if (q == k) {
+            const uint32_t sl = b + mbcnt64(m[k]);
+            if (sl < capacity) {
+                items[k][sl] = value;
+                slot = sl;
+            }
+        }
+    }
+    return slot;
+}
+
+// Workgroup queue push to K queues: as wave_push_multi, but the waves' ballot counts are combined
+// in LDS and one thread issues one atomic per queue for the whole workgroup (16x fewer atomics on
+// the shared counters at 1024 threads). Every thread of the workgroup calls it (it synchronises);
+// `lds_cnt` holds K x 16 words, `lds_base` K words. A lane's slot is its queue's workgroup base +
+// the counts of the waves before it + its rank in its wave (mbcnt). Returns the slot or ~0u.
+template <int K>
+__device__ __forceinline__ uint32_t block_push_multi(uint32_t* const* items, uint32_t* counters, uint32_t capacity,
+                                                     uint32_t value, int q, uint32_t* overflow,
+                                                     uint32_t (*lds_cnt)[16], uint32_t* lds_base) {
+    const uint32_t wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    uint64_t mine = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint64_t m = __ballot(q == k);
+        if (q == k) mine = m;
+        if (lane_id() == 0) lds_cnt[k][wave] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t dropped = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            uint32_t t = 0;
+            for (uint32_t w = 0; w < nwaves; ++w) {
+                const uint32_t c = lds_cnt[k][w];
+                lds_cnt[k][w] = t;
+                t += c;
+            }
+            const uint32_t base = t ? atomicAdd(&counters[k], t) : 0u;
+            lds_base[k] = base;
+            const uint32_t keep = base >= capacity ? 0u : (capacity - base < t ? capacity - base : t);
+            dropped += t - keep;
+        }
+        if (dropped) atomicAdd(overflow, dropped);
+    }
+    __syncthreads();
+    uint32_t slot = 0xFFFFFFFFu;
+    for (int k = 0; k < K; ++k)
+        if (q == k) {
+            const uint32_t sl = lds_base[k] + lds_cnt[k][wave] + mbcnt64(mine);
+            if (sl < capacity) {
+                items[k][sl] = value;
+                slot = sl;
+            }
+        }
+    return slot;
+}
+
+// Workgroup slot reservation on one counter (block_push_multi without the item store): lanes with
+// pred get consecutive slots; a slot at or past `capacity` is dropped, counted in *overflow and
+// returned as ~0u. Every thread of the workgroup calls it.
+__device__ __forceinline__ uint32_t block_reserve(uint32_t* counter, uint32_t capacity, bool pred, uint32_t* overflow,
+                                                  uint32_t (*lds_cnt)[16], uint32_t* lds_base) {
+    const uint32_t wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    const uint64_t m = __ballot(pred);
+    if (lane_id() == 0) lds_cnt[0][wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t w = 0; w < nwaves; ++w) {
+            const uint32_t c = lds_cnt[0][w];
+            lds_cnt[0][w] = t;
+            t += c;
+        }
+        const uint32_t base = t ? atomicAdd(counter, t) : 0u;
+        lds_base[0] = base;
+        const uint32_t keep = base >= capacity ? 0u : (capacity - base < t ? capacity - base : t);
+        if (t > keep) atomicAdd(overflow, t - keep);
+    }
+    __syncthreads();
+    if (!pred) return 0xFFFFFFFFu;
+    const uint32_t sl = lds_base[0] + lds_cnt[0][wave] + mbcnt64(m);
+    return sl < capacity ? sl : 0xFFFFFFFFu;
+}
+
+// Traces items [0, n) with the slice feed and the wave schedule of wave_traverse (one source for
+// every lane). load(item, &tag) gives an item's ray and a per-lane tag; done(item, tag, tr) runs
+// once per item when its traversal has ended (divergent). Called by every lane of the wave.
+template <class Src, class Load, class Done>
+__device__ __forceinline__ void trace_items(uint32_t n, uint32_t slice, int n_nodes, const Src& src, bool anyhit,
+                                            int refill, int leaf_batch, Load load, Done done) {
+    constexpr uint32_t kIdleItem = 0xFFFFFFFFu;
+    WaveFeed f;
+    feed_first(f, n, slice);
+    uint32_t item = kIdleItem, tag = 0;
+    Traversal tr;
+    tr.node = -1;
+    tr.prim = -1;
+    int leaf = -1;
+    for (;;) {
+        const uint64_t idle = __ballot(item == kIdleItem);
+        if (idle != 0 && f.cursor >= f.end && !f.dry) feed_next(f, n);
+        if (idle != 0 && f.cursor < f.end) {
+            const uint32_t cand = f.cursor + mbcnt64(idle);
+            if (item == kIdleItem && cand < f.end) {
+                item = cand;
+                tr = traversal_begin(load(item, &tag));
+                if (n_nodes <= 0) tr.node = -1;
+            }
+            f.cursor = min(f.cursor + (uint32_t)__popcll(idle), f.end);
+        }
+        if (__ballot(item != kIdleItem) == 0) {
+            if (f.dry) break;
+            continue;
+        }
+        wave_traverse(n_nodes, src, tr, leaf, anyhit, (f.dry && f.cursor >= f.end) ? 1 : refill, leaf_batch);
+        if (item != kIdleItem && tr.node < 0 && leaf < 0) {
+            done(item, tag, tr);
+            item = kIdleItem;
+        }
+    }
+}
+
 // `lds` and `lds_prims` must point into LDS (__shared__ memory).
 __device__ __forceinline__ LdsNodes lds_nodes_of(const SceneView& s, const Node* lds, int n_lds,
                                                  const float4* lds_prims = nullptr) {

@@ -63,6 +63,7 @@ struct PersistentTuning {
     int refill = 0;        // traversal refill threshold in lanes of 64 (0 -> 32)
     int leaf_batch = 0;    // parked leaves that trigger a wave's leaf-test round (0 -> 16)
     int dir_bins = 0;      // survivor grouping per block round: 1 none, 8 direction octants (0 -> 1)
+    int shade_block = 0;   // threads per shade workgroup (= compaction round): 128..1024 (0 -> 512)
 };
 
 struct PersistentResult {
@@ -82,12 +83,23 @@ struct JobDev {
     DeviceFilm film;
 };
 
+// Survivors and shadow rays are compacted per segment: shade round r (a workgroup's items) uses the
+// counters of segment r mod kSegs, so concurrent rounds spread their atomics over kSegs addresses
+// instead of serialising on one. Segment s compacts into its own window of the output (its size is
+// the number of items of the rounds that map to s), and the next stage finds item i in segment s
+// with seg_prefix[s] <= i < seg_prefix[s + 1] at position seg_base[s] + i - seg_prefix[s].
+constexpr int kSegs = 16;
+
 // Device control block.
 struct PState {
     unsigned long long next_work, total_work, it_work_base, pad64;
     uint32_t n_live, n_jobs, have_prev, iterations;
     uint32_t it_live, it_new, it_act, done;
-    uint32_t n_next, n_shadow, pad[2];
+    uint32_t n_next, n_shadow, shade_block, pad;   // n_next, n_shadow: totals of this iteration (set by shadow)
+    uint32_t seg_cnt[kSegs];          // survivors per segment (this iteration)
+    uint32_t sh_cnt[kSegs];           // shadow rays per segment (this iteration)
+    uint32_t in_prefix[kSegs + 1];    // where this iteration's survivor items are (set by plan)
+    uint32_t in_base[kSegs];
 };
 
 // One buffer of dense per-item path state (two of them alternate by iteration).

@@ -23,6 +23,7 @@ constexpr int kDefaultLdsKiB = 64;
 constexpr int kDefaultRefill = 32;
 constexpr int kDefaultLeafBatch = 16;
 constexpr int kDefaultDirBins = 1;
+constexpr int kDefaultShadeBlock = 512;
 constexpr uint32_t kIdle = 0xFFFFFFFFu;
 constexpr uint32_t kFinalFlag = 0x80000000u;
 
@@ -101,7 +102,15 @@ __device__ __forceinline__ PathState new_path(const PCtx& c, uint32_t n_jobs, un
 template <bool SINGLE>
 __device__ __forceinline__ void stage_lds(const PCtx& c, Node* lds, LdsPlan* sp) {
     if constexpr (SINGLE) {
-        if (threadIdx.x == 0) *sp = c.plan0;
+        // Word by word with constant offsets: an aggregate copy of the by-value argument can
+        // be lowered through private memory.
+        constexpr int kWords = (int)(sizeof(LdsPlan) / sizeof(int));
+        const int* pw = reinterpret_cast<const int*>(&c.plan0);
+        int* dw = reinterpret_cast<int*>(sp);
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int i = 0; i < kWords; ++i) dw[i] = pw[i];
+        }
     } else {
         if (threadIdx.x == 0) *sp = *c.plan;
     }
@@ -134,43 +143,25 @@ __device__ __forceinline__ void count_done(uint32_t* job_done, bool pred, uint32
     }
 }
 
-// Rays of a traversal kernel are dealt to the waves in slices of kFeedSlice consecutive items,
-// round-robin: wave w takes slices w, w + W, w + 2W, ... of the W waves of the grid. Each slice
-// is spatially coherent (consecutive items are neighbouring pixels or their survivors) while every
-// wave samples the whole image, so the grid stays balanced however the cost of a ray varies across
-// the image, without atomics. Wave-uniform state.
-constexpr uint32_t kFeedSlice = 128;
-
-struct WaveFeed {
-    uint32_t cursor = 0, end = 0;   // rest of the current slice [cursor, end)
-    uint32_t slice = 0, stride = 0;
-    bool dry = false;               // no slice left
-};
-
-__device__ __forceinline__ void feed_first(WaveFeed& f, uint32_t n) {
-    const uint32_t wpb = blockDim.x >> 6;
-    f.stride = gridDim.x * wpb;
-    f.slice = blockIdx.x * wpb + (threadIdx.x >> 6);
-    f.cursor = min(f.slice * kFeedSlice, n);
-    f.end = min(f.cursor + kFeedSlice, n);
-    f.dry = f.cursor >= n;
-}
-
-__device__ __forceinline__ void feed_next(WaveFeed& f, uint32_t n) {
-    f.slice += f.stride;
-    const unsigned long long c = (unsigned long long)f.slice * kFeedSlice;
-    if (c >= n) {
-        f.dry = true;
-        f.cursor = f.end = n;
-    } else {
-        f.cursor = (uint32_t)c;
-        f.end = min(f.cursor + kFeedSlice, n);
+// Output window of each segment for a shade pass over n items in rounds of B: segment s holds the
+// items of the rounds r with r mod kSegs == s; windows follow each other in segment order.
+__device__ __forceinline__ void seg_windows(uint32_t n, uint32_t B, uint32_t* base) {
+    const uint32_t full = n / B, part = n - full * B;
+    uint32_t acc = 0;
+    for (int s = 0; s < kSegs; ++s) {
+        base[s] = acc;
+        const uint32_t rounds = full > (uint32_t)s ? (full - 1 - (uint32_t)s) / kSegs + 1 : 0u;
+        acc += rounds * B + ((part > 0 && full % kSegs == (uint32_t)s) ? part : 0u);
     }
 }
 
-// True when no wave of this block has a slice (the block returns before staging LDS).
-__device__ __forceinline__ bool block_idle(uint32_t n) {
-    return (unsigned long long)blockIdx.x * (blockDim.x >> 6) * kFeedSlice >= n;
+// Position of item i of a segmented list (largest s with prefix[s] <= i).
+__device__ __forceinline__ uint32_t seg_pos(const uint32_t* prefix, const uint32_t* base, uint32_t i) {
+    int s = 0;
+#pragma unroll
+    for (int step = kSegs / 2; step > 0; step >>= 1)
+        if (prefix[s + step] <= i) s += step;
+    return base[s] + (i - prefix[s]);
 }
 
 __global__ void pw_init_kernel(PState* st) {
@@ -186,15 +177,28 @@ __global__ void pw_admit_kernel(PState* st, JobDev* jobs, uint32_t* job_done, Jo
     st->done = 0;
 }
 
-__global__ void pw_plan_kernel(PState* st, DevCounters* ctr, uint32_t cap) {
+__global__ void pw_plan_kernel(PState* st, DevCounters* ctr, uint32_t cap, uint32_t shade_block) {
     if (st->have_prev) {
+        // Fold the previous iteration: its survivors become this iteration's first items, found
+        // through the segment windows of its shade pass.
+        uint32_t next = 0, shadow = 0;
+        uint32_t win[kSegs];
+        seg_windows(st->it_act, st->shade_block, win);
+        for (int s = 0; s < kSegs; ++s) {
+            st->in_prefix[s] = next;
+            st->in_base[s] = win[s];
+            next += st->seg_cnt[s];
+            shadow += st->sh_cnt[s];
+        }
+        st->in_prefix[kSegs] = next;
         ctr->extension_rays += st->it_act;
-        ctr->shadow_rays += st->n_shadow;
-        if (st->n_shadow > ctr->high_water[3]) ctr->high_water[3] = st->n_shadow;
+        ctr->shadow_rays += shadow;
+        if (shadow > ctr->high_water[3]) ctr->high_water[3] = shadow;
         if (st->it_act > ctr->high_water[4]) ctr->high_water[4] = st->it_act;
-        st->n_live = st->n_next;
+        st->n_live = next;
     }
     st->have_prev = 1;
+    st->shade_block = shade_block;
     const unsigned long long rem = st->total_work - st->next_work;
     const uint32_t room = cap - st->n_live;
     const uint32_t n_new = (uint32_t)((unsigned long long)room < rem ? (unsigned long long)room : rem);
@@ -205,6 +209,10 @@ __global__ void pw_plan_kernel(PState* st, DevCounters* ctr, uint32_t cap) {
     st->next_work += n_new;
     st->n_next = 0;
     st->n_shadow = 0;
+    for (int s = 0; s < kSegs; ++s) {
+        st->seg_cnt[s] = 0;
+        st->sh_cnt[s] = 0;
+    }
     if (st->it_act > 0) st->iterations += 1;
     st->done = st->it_act == 0 ? 1u : 0u;
 }
@@ -218,7 +226,10 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
     const uint32_t n_act = st->it_act, n_live = st->it_live, n_jobs = st->n_jobs;
     const unsigned long long wbase = st->it_work_base;
     if (block_idle(n_act)) return;
-    stage_lds<SINGLE>(c, lds, &sp);
+    __shared__ uint32_t pre[kSegs + 1], bas[kSegs];
+    if (threadIdx.x <= kSegs) pre[threadIdx.x] = st->in_prefix[threadIdx.x];
+    if (threadIdx.x < kSegs) bas[threadIdx.x] = st->in_base[threadIdx.x];
+    stage_lds<SINGLE>(c, lds, &sp);   // synchronises
     WaveFeed f;
     feed_first(f, n_act);
     LdsNodes src{};
@@ -241,12 +252,13 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
                 item = cand;
                 Ray r;
                 if (item < n_live) {
-                    const float4 o = in.ro[item], d = in.rd[item];
+                    const uint32_t p = seg_pos(pre, bas, item);
+                    const float4 o = in.ro[p], d = in.rd[p];
                     r.o = Vec3{o.x, o.y, o.z};
                     r.d = Vec3{d.x, d.y, d.z};
                     r.tmin = kRayEps;
                     r.tmax = kInf;
-                    if constexpr (!SINGLE) sc = c.jobs[in.kw[item].z].scene;
+                    if constexpr (!SINGLE) sc = c.jobs[in.kw[p].z].scene;
                 } else {
                     uint32_t jid, sidx;
                     r = new_path<SINGLE>(c, n_jobs, wbase + (item - n_live), &jid, &sidx).ray;
@@ -279,6 +291,27 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
     wave_add_u32(&c.ctr->invalid_hits, inv);
 }
 
+// Inputs of one shade item: its hit and, for a survivor of the previous iteration, its state.
+struct ShadeIn {
+    float2 hv;
+    float4 o, d, t, l;
+    uint4 kw;
+};
+
+__device__ __forceinline__ void shade_load(const PCtx& c, const PathBuf& in, const uint32_t* pre, const uint32_t* bas,
+                                           uint32_t item, uint32_t n, uint32_t n_live, ShadeIn* x) {
+    if (item >= n) return;
+    x->hv = c.hit[item];
+    if (item < n_live) {
+        const uint32_t p = seg_pos(pre, bas, item);
+        x->o = in.ro[p];
+        x->d = in.rd[p];
+        x->t = in.T[p];
+        x->l = in.L[p];
+        x->kw = in.kw[p];
+    }
+}
+
 // Classify + shade of items [0, it_act). Survivors are compacted into `out`; slots for survivors
 // and shadow rays are reserved once per block round. Inside a round the survivors are grouped by
 // the octant of their new direction (dir_bins = 8), each group in item order, so the waves of the
@@ -287,14 +320,25 @@ template <bool SINGLE>
 __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, PathBuf out) {
     __shared__ uint32_t cnt[2][10][16];   // per round parity: 8 direction bins, shadow, finished
     __shared__ uint32_t bases[2][2];
+    __shared__ uint32_t pre[kSegs + 1], bas[kSegs], win[kSegs];
     PState* st = c.st;
     const uint32_t n = st->it_act, n_live = st->it_live, n_jobs = st->n_jobs;
     const unsigned long long wbase = st->it_work_base;
     const uint32_t wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
     const uint32_t stride = gridDim.x * blockDim.x;
+    if (blockIdx.x * blockDim.x >= n) return;   // no round for this workgroup
+    if (threadIdx.x <= kSegs) pre[threadIdx.x] = st->in_prefix[threadIdx.x];
+    if (threadIdx.x < kSegs) bas[threadIdx.x] = st->in_base[threadIdx.x];
+    if (threadIdx.x == 0) seg_windows(n, blockDim.x, win);
+    __syncthreads();
     int round = 0;
     for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n; b0 += stride, round ^= 1) {
+        const uint32_t seg = (b0 / blockDim.x) % kSegs;
         const uint32_t item = b0 + threadIdx.x;
+        // Loading the next round's inputs ahead (a software pipeline) measured slower at the
+        // default 512-thread blocks: the extra live registers cost more occupancy than it hid.
+        ShadeIn cur{};
+        shade_load(c, in, pre, bas, item, n, n_live, &cur);
         bool live = false, fin = false;
         int has = 0;
         uint32_t jid = 0, sidx = 0;
@@ -303,10 +347,10 @@ __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, Path
         Ray sh{};
         Vec3 contrib{0.0f, 0.0f, 0.0f};
         if (item < n) {
-            const float2 hv = c.hit[item];
+            const float2 hv = cur.hv;
             if (item < n_live) {
-                const float4 o = in.ro[item], d = in.rd[item], t = in.T[item], l = in.L[item];
-                const uint4 kw = in.kw[item];
+                const float4 o = cur.o, d = cur.d, t = cur.t, l = cur.l;
+                const uint4 kw = cur.kw;
                 ps.ray.o = Vec3{o.x, o.y, o.z};
                 ps.ray.d = Vec3{d.x, d.y, d.z};
                 ps.ray.tmin = kRayEps;
@@ -366,10 +410,9 @@ __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, Path
                 t1 += b;
                 t2 += d;
             }
-            bases[round][0] = t0 ? atomicAdd(&st->n_next, t0) : 0u;
-            bases[round][1] = t1 ? atomicAdd(&st->n_shadow, t1) : 0u;
-            if constexpr (SINGLE)
-                if (t2) atomicAdd(&c.job_done[0], t2);
+            bases[round][0] = win[seg] + (t0 ? atomicAdd(&st->seg_cnt[seg], t0) : 0u);
+            bases[round][1] = win[seg] + (t1 ? atomicAdd(&st->sh_cnt[seg], t1) : 0u);
+            (void)t2;   // a single job's finished paths are counted once per iteration (shadow kernel)
         }
         __syncthreads();
         uint32_t o = 0;
@@ -398,7 +441,33 @@ template <bool SINGLE>
 __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
     extern __shared__ Node lds[];
     __shared__ LdsPlan sp;
-    const uint32_t n = c.st->n_shadow;
+    __shared__ uint32_t pre[kSegs + 1], win[kSegs], shc[kSegs], svc[kSegs];
+    PState* st = c.st;
+    if (threadIdx.x < kSegs) {
+        shc[threadIdx.x] = st->sh_cnt[threadIdx.x];
+        svc[threadIdx.x] = st->seg_cnt[threadIdx.x];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // The shadow rays of this iteration, segment by segment (windows of the shade pass).
+        uint32_t acc = 0, next = 0;
+        for (int s = 0; s < kSegs; ++s) {
+            pre[s] = acc;
+            acc += shc[s];
+            next += svc[s];
+        }
+        pre[kSegs] = acc;
+        seg_windows(st->it_act, st->shade_block, win);
+        if (blockIdx.x == 0) {
+            st->n_next = next;
+            st->n_shadow = acc;
+            // A single job: every path that did not survive this iteration is finished once this
+            // kernel is done (a pending shadow ray is the last thing it waits for).
+            if constexpr (SINGLE) c.job_done[0] += st->it_act - next;
+        }
+    }
+    __syncthreads();
+    const uint32_t n = pre[kSegs];
     if (block_idle(n)) return;
     stage_lds<SINGLE>(c, lds, &sp);
     WaveFeed f;
@@ -409,7 +478,7 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
         src = src_of(c.scene0, lds, &sp, 0);
         nn = c.scene0.n_nodes;
     }
-    uint32_t item = kIdle, tag = 0, ndone = 0;
+    uint32_t item = kIdle, pos = 0, tag = 0;
     Traversal tr;
     tr.node = -1;
     tr.prim = -1;
@@ -421,7 +490,8 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
             const uint32_t cand = f.cursor + mbcnt64(idle);
             if (item == kIdle && cand < f.end) {
                 item = cand;
-                const float4 o = c.SQ.o[item], d = c.SQ.d[item];
+                pos = seg_pos(pre, win, item);
+                const float4 o = c.SQ.o[pos], d = c.SQ.d[pos];
                 tag = (uint32_t)__float_as_int(d.w);
                 Ray r;
                 r.o = Vec3{o.x, o.y, o.z};
@@ -449,7 +519,7 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
             fin = (tag & kFinalFlag) != 0;
             if constexpr (!SINGLE) jid = tag & ~kFinalFlag;
             if (tr.prim < 0) {
-                const float4 cc = c.SQ.c[item];
+                const float4 cc = c.SQ.c[pos];
                 const uint32_t dest = (uint32_t)__float_as_int(cc.w);
                 float4* dst = fin ? &job_of<SINGLE>(c, jid).sbuf[dest] : &out.L[dest];
                 float4 l = *dst;
@@ -460,10 +530,8 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
             }
             item = kIdle;
         }
-        if constexpr (SINGLE) ndone += fin ? 1u : 0u;
-        else count_done(c.job_done, fin, jid);
+        if constexpr (!SINGLE) count_done(c.job_done, fin, jid);
     }
-    if constexpr (SINGLE) wave_add_u32(&c.job_done[0], ndone);
 }
 
 // Adds a finished job's samples to its film, per pixel in sample order.
@@ -504,6 +572,7 @@ struct PersistentEngine::Impl {
     int refill = kDefaultRefill;
     int leaf_batch = kDefaultLeafBatch;
     int dir_bins = kDefaultDirBins;
+    int shade_block = kDefaultShadeBlock;
     std::vector<void*> pool_allocs;
     PState* st = nullptr;
     PState* st_host = nullptr;       // pinned mirror
@@ -529,12 +598,12 @@ struct PersistentEngine::Impl {
     struct GraphKey {
         bool single = false;
         uint64_t scene_gen = 0;
-        int block = 0, refill = 0, leaf_batch = 0, dir_bins = 0, cap = 0, iters = 0;
+        int block = 0, refill = 0, leaf_batch = 0, dir_bins = 0, shade_block = 0, cap = 0, iters = 0;
         size_t lds_bytes = 0;
         const void* ctr = nullptr;
         bool operator==(const GraphKey& o) const {
             return single == o.single && scene_gen == o.scene_gen && block == o.block && refill == o.refill &&
-                   leaf_batch == o.leaf_batch && dir_bins == o.dir_bins && cap == o.cap && iters == o.iters && lds_bytes == o.lds_bytes &&
+                   leaf_batch == o.leaf_batch && dir_bins == o.dir_bins && shade_block == o.shade_block && cap == o.cap && iters == o.iters && lds_bytes == o.lds_bytes &&
                    ctr == o.ctr;
         }
     } graph_key;
@@ -601,6 +670,9 @@ void PersistentEngine::ensure_pool(const PersistentTuning& t) {
     m.leaf_batch = t.leaf_batch > 0 ? t.leaf_batch : kDefaultLeafBatch;
     if (m.leaf_batch > 64) throw std::runtime_error("persistent: leaf batch must be 1..64 lanes");
     m.dir_bins = t.dir_bins > 0 ? t.dir_bins : kDefaultDirBins;
+    m.shade_block = t.shade_block > 0 ? t.shade_block : kDefaultShadeBlock;
+    if (m.shade_block != 128 && m.shade_block != 256 && m.shade_block != 512 && m.shade_block != 1024)
+        throw std::runtime_error("persistent: shade block size must be 128, 256, 512 or 1024");
     if (m.dir_bins != 1 && m.dir_bins != 8) throw std::runtime_error("persistent: direction bins must be 1 or 8");
     m.ctx.refill = m.refill;
     m.ctx.leaf_batch = m.leaf_batch;
@@ -800,7 +872,7 @@ PersistentEngine::LaunchPlan PersistentEngine::make_plan(bool single) {
             check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.lds_bytes),
                   "hipFuncSetAttribute");
     L.gx = grid_for(fx, m.block, L.lds_bytes, m.cus);
-    L.gs = grid_for(fs, m.block, 0, m.cus);
+    L.gs = grid_for(fs, m.shade_block, 0, m.cus);
     L.gw = grid_for(fw, m.block, L.lds_bytes, m.cus);
     return L;
 }
@@ -815,13 +887,13 @@ void PersistentEngine::launch_iteration(const LaunchPlan& L, int par, hipEvent_t
     const PathBuf& in = m.S[par & 1];
     const PathBuf& out = m.S[(par & 1) ^ 1];
     if (ev) check(hipEventRecord(ev[0], st), "hipEventRecord");
-    pw_plan_kernel<<<1, 1, 0, st>>>(m.st, m.ctr, c.cap);
+    pw_plan_kernel<<<1, 1, 0, st>>>(m.st, m.ctr, c.cap, (uint32_t)m.shade_block);
     if (ev) check(hipEventRecord(ev[1], st), "hipEventRecord");
     if (L.single) pw_extend_kernel<true><<<L.gx, m.block, L.lds_bytes, st>>>(c, in);
     else pw_extend_kernel<false><<<L.gx, m.block, L.lds_bytes, st>>>(c, in);
     if (ev) check(hipEventRecord(ev[2], st), "hipEventRecord");
-    if (L.single) pw_shade_kernel<true><<<L.gs, m.block, 0, st>>>(c, in, out);
-    else pw_shade_kernel<false><<<L.gs, m.block, 0, st>>>(c, in, out);
+    if (L.single) pw_shade_kernel<true><<<L.gs, m.shade_block, 0, st>>>(c, in, out);
+    else pw_shade_kernel<false><<<L.gs, m.shade_block, 0, st>>>(c, in, out);
     if (ev) check(hipEventRecord(ev[3], st), "hipEventRecord");
     if (L.single) pw_shadow_kernel<true><<<L.gw, m.block, L.lds_bytes, st>>>(c, out);
     else pw_shadow_kernel<false><<<L.gw, m.block, L.lds_bytes, st>>>(c, out);
@@ -849,6 +921,7 @@ void PersistentEngine::run(int launches) {
     k.refill = m.refill;
     k.leaf_batch = m.leaf_batch;
     k.dir_bins = m.dir_bins;
+    k.shade_block = m.shade_block;
     k.cap = cap_;
     k.iters = m.graph_iters;
     k.ctr = m.ctx.ctr;
@@ -1066,6 +1139,12 @@ PersistentTuning autotune_persistent(const PersistentJob* jobs, int n_jobs, hipS
             t.leaf_batch = lb;
             time(t);
         }
+    base = best;
+    for (int sb : {256, 1024}) {
+        PersistentTuning t = base;
+        t.shade_block = sb;
+        time(t);
+    }
     base = best;
     for (int nb : {1, 8}) {
         if (nb == (base.dir_bins > 0 ? base.dir_bins : kDefaultDirBins)) continue;

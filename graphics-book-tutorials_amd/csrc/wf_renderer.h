@@ -28,6 +28,7 @@ struct RenderParams {
     int refill = 0;              // persistent: traversal refill threshold (lanes)
     int leaf_batch = 0;          // persistent: parked leaves per leaf-test round
     int dir_bins = 0;            // persistent: survivor grouping (1 or 8 direction octants)
+    int shade_block = 0;         // persistent: threads per shade workgroup
     int lds_nodes = -1;          // nodes staged in LDS per workgroup (-1 = default)
 };
 

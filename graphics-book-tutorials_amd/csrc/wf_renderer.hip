@@ -148,6 +148,7 @@ void GpuRenderer::run_passes(int first_sample, int count, const RenderParams& p,
             tune.refill = p.refill;
             tune.leaf_batch = p.leaf_batch;
             tune.dir_bins = p.dir_bins;
+            tune.shade_block = p.shade_block;
             gpu::PersistentResult pr;
             m.pe->render(&job, 1, tune, m.d_ctr, m.stream, &pr, p.stage_timing ? &st : nullptr);
             iterations = pr.iterations;

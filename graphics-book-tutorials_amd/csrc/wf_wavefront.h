@@ -49,6 +49,9 @@ private:
     uint32_t* trace_dev_ = nullptr;
     ScanWorkspace ws_;
     hipEvent_t events_[2] = {nullptr, nullptr};
+    // Persistent grid sizes of the traversal kernels for the LDS size they were computed for.
+    size_t trace_lds_ = (size_t)-1;
+    int extend_grid_ = 0, shadow_grid_ = 0, cus_ = 0;
     std::vector<void*> allocs_;
     uint64_t bytes_ = 0;
 };

@@ -30,29 +30,54 @@ __global__ __launch_bounds__(kThreads) void wf_init_kernel(SceneView s, PathPool
     active[i] = (uint32_t)i;
 }
 
-__global__ __launch_bounds__(kThreads) void wf_extend_kernel(SceneView s, PathPool P, const uint32_t* active,
-                                                             uint32_t n_active, int n_lds) {
-    extern __shared__ Node lds_nodes[];
-    stage_nodes(lds_nodes, s.nodes, n_lds);
-    const uint32_t idx = blockIdx.x * kThreads + threadIdx.x;
-    if (idx >= n_active) return;
-    const uint32_t i = active[idx];
-    const float4 o = P.ro[i], d = P.rd[i];
-    Ray r;
-    r.o = Vec3{o.x, o.y, o.z};
-    r.d = Vec3{d.x, d.y, d.z};
-    r.tmin = kRayEps;
-    r.tmax = kInf;
-    Hit h;
-    intersect_with(s, lds_nodes_of(s, lds_nodes, n_lds), r, &h, false);
-    P.hit[i] = make_float2(h.t, __int_as_float(h.prim));
+__device__ __forceinline__ uint32_t clamp_count(uint32_t c, uint32_t cap) { return c < cap ? c : cap; }
+
+// Traversal kernels run on a persistent grid (occupancy x CUs) with the wave schedule of
+// trace_items; each workgroup stages the BVH top (and, when the whole tree is staged and they fit,
+// the primitive records) into LDS once.
+constexpr int kTraceThreads = 512;
+constexpr int kTraceRefill = 32;
+constexpr int kTraceLeafBatch = 16;
+constexpr uint32_t kTraceSlice = 64;
+
+__device__ __forceinline__ LdsNodes stage_top(const SceneView& s, Node* lds, int n_lds, int n_lds_prims) {
+    for (int k = threadIdx.x; k < n_lds; k += blockDim.x) lds[k] = s.nodes[k];
+    float4* lprim = reinterpret_cast<float4*>(lds + n_lds);
+    for (int k = threadIdx.x; k < n_lds_prims; k += blockDim.x) lprim[k] = lds_prim_record(s, k);
+    __syncthreads();
+    return lds_nodes_of(s, lds, n_lds, n_lds_prims > 0 ? lprim : nullptr);
 }
 
-__global__ __launch_bounds__(kThreads) void wf_classify_kernel(SceneView s, PathPool P, const uint32_t* active,
-                                                               uint32_t n_active, uint32_t* alive, uint32_t* q0,
-                                                               uint32_t* q1, uint32_t* q2, uint32_t* cnt,
-                                                               uint32_t cap, int max_depth, DevCounters* ctr) {
-    const uint32_t idx = blockIdx.x * kThreads + threadIdx.x;
+__global__ __launch_bounds__(kTraceThreads) void wf_extend_kernel(SceneView s, PathPool P, const uint32_t* active,
+                                                                  uint32_t n_active, int n_lds, int n_lds_prims) {
+    extern __shared__ Node lds_nodes[];
+    if (block_idle(n_active, kTraceSlice)) return;
+    const LdsNodes src = stage_top(s, lds_nodes, n_lds, n_lds_prims);
+    trace_items(
+        n_active, kTraceSlice, s.n_nodes, src, false, kTraceRefill, kTraceLeafBatch,
+        [&](uint32_t idx, uint32_t* slot) {
+            const uint32_t i = active[idx];
+            *slot = i;
+            const float4 o = P.ro[i], d = P.rd[i];
+            Ray r;
+            r.o = Vec3{o.x, o.y, o.z};
+            r.d = Vec3{d.x, d.y, d.z};
+            r.tmin = kRayEps;
+            r.tmax = kInf;
+            return r;
+        },
+        [&](uint32_t, uint32_t i, const Traversal& tr) { P.hit[i] = make_float2(tr.r.tmax, __int_as_float(tr.prim)); });
+}
+
+// Classify and shade_lambert push with one atomic per queue per 1024-thread workgroup.
+constexpr int kPushThreads = 1024;
+
+__global__ __launch_bounds__(kPushThreads) void wf_classify_kernel(SceneView s, PathPool P, const uint32_t* active,
+                                                                   uint32_t n_active, uint32_t* alive, uint32_t* q0,
+                                                                   uint32_t* q1, uint32_t* q2, uint32_t* cnt,
+                                                                   uint32_t cap, int max_depth, DevCounters* ctr) {
+    __shared__ uint32_t lds_cnt[3][16], lds_base[3];
+    const uint32_t idx = blockIdx.x * kPushThreads + threadIdx.x;
     const bool valid = idx < n_active;
     uint32_t i = 0, inv = 0;
     int mat = -1;
@@ -72,38 +97,39 @@ __global__ __launch_bounds__(kThreads) void wf_classify_kernel(SceneView s, Path
         alive[i] = mat >= 0 ? 1u : 0u;
     }
     uint32_t* qs[3] = {q0, q1, q2};
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        const bool mine = valid && mat == q;
-        const uint32_t slot = wave_push(qs[q], &cnt[q], cap, i, mine, &cnt[kCntOverflow]);
-        if (mine && slot == 0xFFFFFFFFu) alive[i] = 0u;   // dropped on overflow: the path ends
-    }
+    const int q = valid ? mat : -1;
+    const uint32_t slot = block_push_multi<3>(qs, cnt, cap, i, q, &cnt[kCntOverflow], lds_cnt, lds_base);
+    if (q >= 0 && slot == 0xFFFFFFFFu) alive[i] = 0u;   // dropped on overflow: the path ends
     wave_add_u32(&ctr->invalid_hits, inv);
 }
 
-__device__ __forceinline__ uint32_t clamp_count(uint32_t c, uint32_t cap) { return c < cap ? c : cap; }
 
-__global__ __launch_bounds__(kThreads) void wf_shade_lambert_kernel(SceneView s, PathPool P, ShadowQueue SQ,
-                                                                    const uint32_t* queue, uint32_t* alive,
-                                                                    uint32_t* cnt, uint32_t cap) {
-    const uint32_t idx = blockIdx.x * kThreads + threadIdx.x;
-    if (idx >= clamp_count(cnt[kCntLambert], cap)) return;
-    const uint32_t i = queue[idx];
-    const Surface f = load_surface(P, i);
-    PathState ps = load_path(P, i, false);
-    Ray sh;
-    Vec3 c;
-    int has;
-    const bool live = shade_lambert(s, ps, f, &sh, &c, &has);
-    store_scatter(P, i, ps);
-    alive[i] = live ? 1u : 0u;
-    const uint32_t slot = wave_reserve(&cnt[kCntShadow], has != 0);
-    if (has && slot < cap) {
+__global__ __launch_bounds__(kPushThreads) void wf_shade_lambert_kernel(SceneView s, PathPool P, ShadowQueue SQ,
+                                                                        const uint32_t* queue, uint32_t* alive,
+                                                                        uint32_t* cnt, uint32_t cap) {
+    __shared__ uint32_t lds_cnt[1][16], lds_base[1];
+    const uint32_t n = clamp_count(cnt[kCntLambert], cap);
+    if (blockIdx.x * kPushThreads >= n) return;   // whole workgroup
+    const uint32_t idx = blockIdx.x * kPushThreads + threadIdx.x;
+    uint32_t i = 0;
+    int has = 0;
+    Ray sh{};
+    Vec3 c{0.0f, 0.0f, 0.0f};
+    if (idx < n) {
+        i = queue[idx];
+        const Surface f = load_surface(P, i);
+        PathState ps = load_path(P, i, false);
+        const bool live = shade_lambert(s, ps, f, &sh, &c, &has);
+        store_scatter(P, i, ps);
+        alive[i] = live ? 1u : 0u;
+    }
+    // Shadow-queue slots; a shadow ray that does not fit is dropped and counted (this bounce loses
+    // its NEE contribution).
+    const uint32_t slot = block_reserve(&cnt[kCntShadow], cap, has != 0, &cnt[kCntOverflow], lds_cnt, lds_base);
+    if (has && slot != 0xFFFFFFFFu) {
         SQ.o[slot] = make_float4(sh.o.x, sh.o.y, sh.o.z, sh.tmax);
         SQ.d[slot] = make_float4(sh.d.x, sh.d.y, sh.d.z, __int_as_float((int)i));
         SQ.c[slot] = make_float4(c.x, c.y, c.z, 0.0f);
-    } else if (has) {
-        atomicAdd(&cnt[kCntOverflow], 1u);   // dropped: this bounce loses its NEE contribution
     }
 }
 
@@ -121,28 +147,35 @@ __global__ __launch_bounds__(kThreads) void wf_shade_specular_kernel(SceneView s
     alive[i] = live ? 1u : 0u;
 }
 
-__global__ __launch_bounds__(kThreads) void wf_shadow_kernel(SceneView s, PathPool P, ShadowQueue SQ,
-                                                             const uint32_t* cnt, uint32_t cap, int n_lds) {
+__global__ __launch_bounds__(kTraceThreads) void wf_shadow_kernel(SceneView s, PathPool P, ShadowQueue SQ,
+                                                                  const uint32_t* cnt, uint32_t cap, int n_lds,
+                                                                  int n_lds_prims) {
     extern __shared__ Node lds_nodes[];
-    stage_nodes(lds_nodes, s.nodes, n_lds);
-    const uint32_t idx = blockIdx.x * kThreads + threadIdx.x;
-    if (idx >= clamp_count(cnt[kCntShadow], cap)) return;
-    const float4 o = SQ.o[idx], d = SQ.d[idx];
-    Ray r;
-    r.o = Vec3{o.x, o.y, o.z};
-    r.d = Vec3{d.x, d.y, d.z};
-    r.tmin = kRayEps;
-    r.tmax = o.w;
-    Hit h;
-    if (!intersect_with(s, lds_nodes_of(s, lds_nodes, n_lds), r, &h, true)) {
-        const uint32_t i = (uint32_t)__float_as_int(d.w);
-        const float4 c = SQ.c[idx];
-        float4 l = P.L[i];
-        l.x += c.x;
-        l.y += c.y;
-        l.z += c.z;
-        P.L[i] = l;
-    }
+    const uint32_t n = clamp_count(cnt[kCntShadow], cap);
+    if (block_idle(n, kTraceSlice)) return;
+    const LdsNodes src = stage_top(s, lds_nodes, n_lds, n_lds_prims);
+    trace_items(
+        n, kTraceSlice, s.n_nodes, src, true, kTraceRefill, kTraceLeafBatch,
+        [&](uint32_t idx, uint32_t* slot) {
+            const float4 o = SQ.o[idx], d = SQ.d[idx];
+            *slot = (uint32_t)__float_as_int(d.w);
+            Ray r;
+            r.o = Vec3{o.x, o.y, o.z};
+            r.d = Vec3{d.x, d.y, d.z};
+            r.tmin = kRayEps;
+            r.tmax = o.w;
+            return r;
+        },
+        [&](uint32_t idx, uint32_t i, const Traversal& tr) {
+            if (tr.prim < 0) {   // unoccluded
+                const float4 c = SQ.c[idx];
+                float4 l = P.L[i];
+                l.x += c.x;
+                l.y += c.y;
+                l.z += c.z;
+                P.L[i] = l;
+            }
+        });
 }
 
 __global__ __launch_bounds__(kThreads) void wf_mark_kernel(const uint32_t* active, uint32_t n_active,
@@ -244,7 +277,26 @@ void WavefrontEngine::run_pass(const DeviceFilm& film, DevCounters* ctr, uint32_
     const uint32_t P = (uint32_t)pixels_;
     const uint32_t cap = (opt.queue_capacity > 0 && (uint32_t)opt.queue_capacity < P) ? (uint32_t)opt.queue_capacity : P;
     const int n_lds = opt.lds_nodes < 0 ? std::min(scene_.n_nodes, kWavefrontLdsNodes) : std::min(scene_.n_nodes, opt.lds_nodes);
-    const size_t lds_bytes = (size_t)n_lds * sizeof(Node);
+    // Primitive records join the nodes in LDS when the whole tree is staged and both fit in 64 KiB.
+    const int n_lds_prims =
+        (n_lds == scene_.n_nodes && (size_t)n_lds * sizeof(Node) + (size_t)scene_.n_prims * 16 <= 64 * 1024)
+            ? scene_.n_prims : 0;
+    const size_t lds_bytes = (size_t)n_lds * sizeof(Node) + (size_t)n_lds_prims * 16;
+    if (lds_bytes != trace_lds_) {
+        if (cus_ == 0) {
+            int dev = 0;
+            check(hipGetDevice(&dev), "hipGetDevice");
+            check(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+        }
+        int per = 0;
+        check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, wf_extend_kernel, kTraceThreads, lds_bytes),
+              "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+        extend_grid_ = std::max(1, per) * cus_;
+        check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, wf_shadow_kernel, kTraceThreads, lds_bytes),
+              "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+        shadow_grid_ = std::max(1, per) * cus_;
+        trace_lds_ = lds_bytes;
+    }
     const int scan_tile = opt.scan_tile > 0 ? opt.scan_tile : kDefaultScanTile;
     const bool timing = stages != nullptr && opt.stage_timing;
     int ev = 0;
@@ -286,16 +338,18 @@ void WavefrontEngine::run_pass(const DeviceFilm& film, DevCounters* ctr, uint32_
     const int max_bounces = max_depth + 2;
     while (n_active > 0 && bounce < max_bounces) {
         zero_words_kernel<<<1, 64, 0, stream>>>(counters_, kCntWords);
-        wf_extend_kernel<<<blocks_for(n_active), kThreads, lds_bytes, stream>>>(scene_, pool_, active_[cur], n_active, n_lds);
+        wf_extend_kernel<<<extend_grid_, kTraceThreads, lds_bytes, stream>>>(scene_, pool_, active_[cur], n_active, n_lds,
+                                                                               n_lds_prims);
         check(hipGetLastError(), "wavefront extend");
         mark(&StageTimes::extend);
-        wf_classify_kernel<<<blocks_for(n_active), kThreads, 0, stream>>>(scene_, pool_, active_[cur], n_active, alive_,
-                                                                          queues_[0], queues_[1], queues_[2], counters_,
-                                                                          cap, max_depth, ctr);
+        const int push_blocks = (int)((n_active + kPushThreads - 1) / kPushThreads);
+        wf_classify_kernel<<<push_blocks, kPushThreads, 0, stream>>>(scene_, pool_, active_[cur], n_active, alive_,
+                                                                     queues_[0], queues_[1], queues_[2], counters_, cap,
+                                                                     max_depth, ctr);
         check(hipGetLastError(), "wavefront classify");
         mark(&StageTimes::classify);
-        wf_shade_lambert_kernel<<<blocks_for(n_active), kThreads, 0, stream>>>(scene_, pool_, sq_, queues_[0], alive_,
-                                                                               counters_, cap);
+        wf_shade_lambert_kernel<<<push_blocks, kPushThreads, 0, stream>>>(scene_, pool_, sq_, queues_[0], alive_,
+                                                                          counters_, cap);
         check(hipGetLastError(), "wavefront shade lambert");
         mark(&StageTimes::shade_lambert);
         wf_shade_specular_kernel<kMetal><<<blocks_for(n_active), kThreads, 0, stream>>>(scene_, pool_, queues_[1],
@@ -306,7 +360,8 @@ void WavefrontEngine::run_pass(const DeviceFilm& film, DevCounters* ctr, uint32_
                                                                                               alive_, counters_, cap);
         check(hipGetLastError(), "wavefront shade dielectric");
         mark(&StageTimes::shade_dielectric);
-        wf_shadow_kernel<<<blocks_for(n_active), kThreads, lds_bytes, stream>>>(scene_, pool_, sq_, counters_, cap, n_lds);
+        wf_shadow_kernel<<<shadow_grid_, kTraceThreads, lds_bytes, stream>>>(scene_, pool_, sq_, counters_, cap, n_lds,
+                                                                              n_lds_prims);
         check(hipGetLastError(), "wavefront shadow");
         mark(&StageTimes::shadow);
         wf_mark_kernel<<<blocks_for(n_active), kThreads, 0, stream>>>(active_[cur], n_active, alive_, flags_);

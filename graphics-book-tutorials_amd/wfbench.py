@@ -35,6 +35,9 @@ import numpy as np
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 import wfamd  # noqa: E402
 
+# Persistent-engine tuning fields (wf_render_params) the autotuner chooses.
+TUNING_KEYS = ("block_size", "pool_paths", "lds_kib", "refill", "leaf_batch", "dir_bins", "shade_block")
+
 # Pinned protocol (the reference pins its values in bench/protocol.json).
 PROTOCOL = {
     "schema_version": 1,
@@ -165,7 +168,7 @@ def stage_profile(scene, mode, spp, seed, depth, tuned=None):
     stage_timing run, src/bench.cpp:166-172)."""
     kw = {}
     if tuned is not None:
-        kw = {k: getattr(tuned, k) for k in ("block_size", "pool_paths", "lds_kib", "refill", "leaf_batch", "dir_bins")}
+        kw = {k: getattr(tuned, k) for k in TUNING_KEYS}
     r = wfamd.Renderer(scene, mode)
     _, _, st = r.render(wfamd.params(spp=spp, seed=seed, max_depth=depth, stage_timing=True, **kw))
     r.close()
@@ -189,9 +192,7 @@ def bench_scene(cfg_id, cfg, quick):
     mk, mk_img, mk_x = run_condition(scene, "megakernel", p, reps, warmup, w, h)
     wv, wv_img, wv_x = run_condition(scene, "wavefront", p, reps, warmup, w, h)
     tuned = wfamd.autotune([(scene, spp, seed, depth)], probe_spp=min(spp, 16))
-    pp = wfamd.params(spp=spp, seed=seed, max_depth=depth, block_size=tuned.block_size, pool_paths=tuned.pool_paths,
-                      lds_kib=tuned.lds_kib, refill=tuned.refill, leaf_batch=tuned.leaf_batch,
-                      dir_bins=tuned.dir_bins)
+    pp = wfamd.params(spp=spp, seed=seed, max_depth=depth, **{k: getattr(tuned, k) for k in TUNING_KEYS})
     pe, pe_img, pe_x = run_condition(scene, "persistent", pp, reps, warmup, w, h)
     quality = []
     rel, _, _ = image_diff(mk_img[0], mk_img[1], spp, wv_img[0], wv_img[1], spp)
@@ -231,8 +232,7 @@ def bench_scene(cfg_id, cfg, quick):
         "persistent_speedup_vs_megakernel": mk["median"] / pe["median"],
         "persistent_equals_megakernel_bitwise": psame == 1.0 and prel == 0.0,
         "mrays_per_s": {"megakernel": mk_x, "wavefront": wv_x, "persistent": pe_x},
-        "tuning": {k: getattr(tuned, k) for k in ("block_size", "pool_paths", "lds_kib", "refill", "leaf_batch",
-                                                   "dir_bins")},
+        "tuning": {k: getattr(tuned, k) for k in TUNING_KEYS},
         "cpu_arm": cpu_x,
         "stages": stages,
     }
@@ -305,8 +305,7 @@ def bench_batch(cfg, quick):
             "rays_equal": rays == mk_rays, "jobs_bitwise_equal_standalone": not mismatches, "mismatched_jobs": mismatches,
             "queue_overflows": st.queue_overflows, "invalid_hits": st.invalid_hits, "iterations": st.iterations,
             "autotune_s": tune_s,
-            "tuning": {k: getattr(tuned, k) for k in ("block_size", "pool_paths", "lds_kib", "refill", "leaf_batch",
-                                                       "dir_bins")},
+            "tuning": {k: getattr(tuned, k) for k in TUNING_KEYS},
             "views": [{"scene": s, "width": w, "height": h, "yaw": yaw, "spp": spp} for s, w, h, yaw, spp in views]}
 
 

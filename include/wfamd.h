@@ -55,6 +55,7 @@ typedef struct wf_render_params {
     int refill;           /* persistent: a wave refetches rays when fewer lanes than this traverse */
     int leaf_batch;       /* persistent: parked leaves that start a wave's leaf-test round */
     int dir_bins;         /* persistent: survivor grouping, 1 none or 8 direction octants */
+    int shade_block;      /* persistent: threads per shade workgroup = compaction round (128..1024) */
 } wf_render_params;
 
 /* RenderStats, include/wf/renderer.hpp:23-37. stage_ms: extend, classify, shade_lambert,
@@ -223,7 +224,7 @@ typedef struct wf_batch_stats {
     uint64_t extension_rays, shadow_rays;
     int64_t invalid_hits, queue_overflows;
     int64_t iterations, graph_launches;
-    int block_size, pool_paths, lds_kib, refill, leaf_batch, dir_bins;
+    int block_size, pool_paths, lds_kib, refill, leaf_batch, dir_bins, shade_block;
     uint64_t device_bytes;
 } wf_batch_stats;
 
