@@ -30,9 +30,9 @@ HBM_PEAK_GBS = 8000.0
 # workload (sphere_grid 1280x720x64, default tuning), each counter scaled by what it reads on a plain
 # copy kernel of known size (FETCH_SIZE x2.0, WRITE_SIZE x1.0); source and commands:
 # profiles/persistent_sphere_grid_1280x720x64_counters.md. Valid for that workload only.
-MEASURED_TRAFFIC = {"sphere_grid 1280x720x64": {"extend": 94.5e6, "shade": 399.8e6, "shadow": 56.6e6}}
+MEASURED_TRAFFIC = {"sphere_grid 1280x720x64": {"extend": 96.5e6, "shade": 400.7e6, "shadow": 56.8e6}}
 # VALU lane utilisation of the traversal kernels (same source).
-MEASURED_LANE_UTIL = {"sphere_grid 1280x720x64": {"extend": 0.605, "shadow": 0.426, "shade": 0.470,
+MEASURED_LANE_UTIL = {"sphere_grid 1280x720x64": {"extend": 0.579, "shadow": 0.454, "shade": 0.516,
                                                    "megakernel": 0.193}}
 
 
