@@ -144,20 +144,27 @@ struct LdsNodes {
     int n_spheres;
     lds_float4* lds_prims;
     int has_lds_prims;
+    __device__ __forceinline__ Node node_lds(int n) const {
+        const f32x4 a = lds[2 * n], b = lds[2 * n + 1];
+        Node out;
+        out.lo = Vec3{a.x, a.y, a.z};
+        out.skip = __float_as_int(a.w);
+        out.hi = Vec3{b.x, b.y, b.z};
+        out.child = __float_as_int(b.w);
+        return out;
+    }
     __device__ __forceinline__ Node node(int n) const {
         Node out;
         if (n < n_lds) {
-            const f32x4 a = lds[2 * n], b = lds[2 * n + 1];
-            out.lo = Vec3{a.x, a.y, a.z};
-            out.skip = __float_as_int(a.w);
-            out.hi = Vec3{b.x, b.y, b.z};
-            out.child = __float_as_int(b.w);
+            out = node_lds(n);
             __builtin_amdgcn_sched_barrier(0);
         } else {
             out = load_node_global(mem, n);
         }
         return out;
     }
+    // Every node of a tree of n_nodes is served from LDS.
+    __device__ __forceinline__ bool all_lds(int n_nodes) const { return n_lds >= n_nodes; }
     // Prim records are 48 bytes: {p0, id}, {p1, radius}, {p2, material}.
     __device__ __forceinline__ bool hit(int slot, const Ray& r, float* t) const {
         const glb_float4* g = (const glb_float4*)prims + 3 * slot;
@@ -213,12 +220,46 @@ __device__ __forceinline__ void trav_leaf(const Src& src, Traversal& tr, int& le
     }
 }
 
-// Runs the wave's live lanes until fewer than `limit` of them are live (limit 1: until all are
-// done). Each round either visits one node in every lane that has no parked leaf, or, once at
-// least `leaf_batch` lanes (or every live lane) have parked a leaf, tests those leaves together.
-template <class Src>
-__device__ __forceinline__ void wave_traverse(int n_nodes, const Src& src, Traversal& tr, int& leaf, bool anyhit,
-                                              int limit, int leaf_batch) {
+// Slab test for a ray whose 1/d components are all finite, equal to hit_box bit for bit:
+//  - no slab bound is NaN: with a finite origin and 1/d finite and nonzero, b - o is finite or
+//    infinite (a box plane may be infinite) and its product with 1/d is never 0 * inf;
+//  - hit_box's swap tn > tf ? tf : tn takes the far plane's bound exactly when 1/d < 0: for
+//    1/d > 0, lo <= hi gives tn <= tf (subtracting o and multiplying by a positive number round
+//    monotonically); for 1/d < 0, tn >= tf, and where they are equal both choices have the same
+//    value up to the sign of a zero, which no comparison sees. neg_x/y/z are those signs, fixed
+//    per ray;
+//  - with no NaN, t0 <= fminf(fminf(fminf(tmax, bx), by), bz) is t0 <= tmax && t0 <= min(bx, by, bz).
+__device__ __forceinline__ bool hit_box_signed(const Vec3& lo, const Vec3& hi, const Vec3& o, const Vec3& inv_d,
+                                               float tmin, float tmax, bool neg_x, bool neg_y, bool neg_z) {
+    const float lx = (lo.x - o.x) * inv_d.x, hx = (hi.x - o.x) * inv_d.x;
+    const float ly = (lo.y - o.y) * inv_d.y, hy = (hi.y - o.y) * inv_d.y;
+    const float lz = (lo.z - o.z) * inv_d.z, hz = (hi.z - o.z) * inv_d.z;
+    const float ax = neg_x ? hx : lx, bx = neg_x ? lx : hx;
+    const float ay = neg_y ? hy : ly, by = neg_y ? ly : hy;
+    const float az = neg_z ? hz : lz, bz = neg_z ? lz : hz;
+    const float t0 = fmaxf(fmaxf(fmaxf(tmin, ax), ay), az);
+    const float t1 = fminf(fminf(bx, by), bz);
+    return t0 <= tmax && t0 <= t1;
+}
+
+// trav_visit for rays with finite 1/d (hit_box_signed). Device trees are checked when they are
+// laid out (every skip is -1 or a node, every child a node or a leaf code), so the next node needs
+// no range check: -1 ends the traversal. ALL_LDS: every node is in LDS.
+template <bool ALL_LDS, class Src>
+__device__ __forceinline__ void trav_visit_signed(const Src& src, Traversal& tr, int& leaf, bool neg_x, bool neg_y,
+                                                  bool neg_z) {
+    Node node;
+    if constexpr (ALL_LDS) node = src.node_lds(tr.node);
+    else node = src.node(tr.node);
+    const bool hit = hit_box_signed(node.lo, node.hi, tr.r.o, tr.inv_d, tr.r.tmin, tr.r.tmax, neg_x, neg_y, neg_z);
+    if (hit && node.child < 0) leaf = ~node.child;   // -(child + 1)
+    tr.node = (hit && node.child >= 0) ? node.child : node.skip;
+}
+
+// MODE 0: trav_visit (any ray); 1: trav_visit_signed; 2: trav_visit_signed with every node in LDS.
+template <int MODE, class Src>
+__device__ __forceinline__ void wave_traverse_loop(int n_nodes, const Src& src, Traversal& tr, int& leaf, bool anyhit,
+                                                   int limit, int leaf_batch, bool neg_x, bool neg_y, bool neg_z) {
     for (;;) {
         const bool parked = leaf >= 0;
         const bool moving = tr.node >= 0 && !parked;
@@ -227,10 +268,37 @@ __device__ __forceinline__ void wave_traverse(int n_nodes, const Src& src, Trave
         if (np + nm < limit || np + nm == 0) break;
         if (np > 0 && (nm == 0 || np >= leaf_batch)) {
             if (parked) trav_leaf(src, tr, leaf, anyhit);
-        } else {
-            if (moving) trav_visit(n_nodes, src, tr, leaf);
+        } else if (moving) {
+            if constexpr (MODE == 0) trav_visit(n_nodes, src, tr, leaf);
+            else trav_visit_signed<MODE == 2>(src, tr, leaf, neg_x, neg_y, neg_z);
         }
     }
+}
+
+// Runs the wave's live lanes until fewer than `limit` of them are live (limit 1: until all are
+// done). Each round either visits one node in every lane that has no parked leaf, or, once at
+// least `leaf_batch` lanes (or every live lane) have parked a leaf, tests those leaves together.
+// The rays do not change inside, so the visit form is chosen once per call for the whole wave:
+// the signed slab test unless a live ray has a 1/d component that is infinite, zero or NaN, or a
+// non-finite origin.
+template <class Src>
+__device__ __forceinline__ void wave_traverse(int n_nodes, const Src& src, Traversal& tr, int& leaf, bool anyhit,
+                                              int limit, int leaf_batch) {
+    limit = __builtin_amdgcn_readfirstlane(limit);
+    leaf_batch = __builtin_amdgcn_readfirstlane(leaf_batch);
+    const bool live = tr.node >= 0 || leaf >= 0;
+    const float inf = __builtin_inff();
+    const Vec3& v = tr.inv_d;
+    const Vec3& o = tr.r.o;
+    const bool finite = fabsf(v.x) < inf && fabsf(v.y) < inf && fabsf(v.z) < inf && v.x != 0.0f && v.y != 0.0f &&
+                        v.z != 0.0f && fabsf(o.x) < inf && fabsf(o.y) < inf && fabsf(o.z) < inf;
+    const bool neg_x = tr.inv_d.x < 0.0f, neg_y = tr.inv_d.y < 0.0f, neg_z = tr.inv_d.z < 0.0f;
+    if (__ballot(live && !finite) != 0)
+        wave_traverse_loop<0>(n_nodes, src, tr, leaf, anyhit, limit, leaf_batch, neg_x, neg_y, neg_z);
+    else if (__ballot(live && !src.all_lds(n_nodes)) == 0)
+        wave_traverse_loop<2>(n_nodes, src, tr, leaf, anyhit, limit, leaf_batch, neg_x, neg_y, neg_z);
+    else
+        wave_traverse_loop<1>(n_nodes, src, tr, leaf, anyhit, limit, leaf_batch, neg_x, neg_y, neg_z);
 }
 
 // Rays of a traversal kernel are dealt to the waves in slices of `size` consecutive items,

@@ -346,6 +346,18 @@ void build_device_layout(Scene& s) {
         }
         s.dnodes[newidx[n]] = d;
     }
+    // Device traversal follows these links without range checks: every skip is -1 or a node, every
+    // child a node or the code of a leaf whose primitives exist, and every box has lo <= hi.
+    for (const Node& d : s.dnodes) {
+        bool ok = d.skip >= -1 && d.skip < nn && d.lo.x <= d.hi.x && d.lo.y <= d.hi.y && d.lo.z <= d.hi.z;
+        if (d.child >= 0) {
+            ok = ok && d.child < nn;
+        } else {
+            const int code = -(d.child + 1);
+            ok = ok && (code >> 2) + (code & 3) + 1 <= np;
+        }
+        if (!ok) throw std::runtime_error("device BVH layout has an invalid link or box");
+    }
     s.dprims.assign(np, Prim{});
     std::vector<int> slot_of(np, -1);
     for (int k = 0; k < np; ++k) {
