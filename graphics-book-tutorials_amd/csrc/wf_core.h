@@ -322,6 +322,39 @@ WF_HD bool hit_box(Vec3 lo, Vec3 hi, Vec3 o, Vec3 inv_d, float tmin, float tmax)
     return t0 <= t1;
 }
 
+// Slab test used by the GPU traversal. For a box with lo <= hi and a ray with a finite origin, 1/d
+// components that are finite and nonzero and a tmin and tmax that are not NaN (slab_signed_ok), it
+// equals hit_box bit for bit:
+//  - no slab bound is NaN: with a finite origin and 1/d finite and nonzero, b - o is finite or
+//    infinite (a box plane may be infinite) and its product with 1/d is never 0 * inf;
+//  - hit_box's swap tn > tf ? tf : tn takes the far plane's bound exactly when 1/d < 0: for
+//    1/d > 0, lo <= hi gives tn <= tf (subtracting o and multiplying by a positive number round
+//    monotonically); for 1/d < 0, tn >= tf, and where they are equal both choices have the same
+//    value up to the sign of a zero, which no comparison sees. neg_x/y/z are those signs, fixed
+//    per ray;
+//  - with no NaN anywhere, t0 <= fminf(fminf(fminf(tmax, bx), by), bz) is
+//    t0 <= tmax && t0 <= min(bx, by, bz).
+WF_HD bool hit_box_signed(const Vec3& lo, const Vec3& hi, const Vec3& o, const Vec3& inv_d, float tmin, float tmax,
+                          bool neg_x, bool neg_y, bool neg_z) {
+    const float lx = (lo.x - o.x) * inv_d.x, hx = (hi.x - o.x) * inv_d.x;
+    const float ly = (lo.y - o.y) * inv_d.y, hy = (hi.y - o.y) * inv_d.y;
+    const float lz = (lo.z - o.z) * inv_d.z, hz = (hi.z - o.z) * inv_d.z;
+    const float ax = neg_x ? hx : lx, bx = neg_x ? lx : hx;
+    const float ay = neg_y ? hy : ly, by = neg_y ? ly : hy;
+    const float az = neg_z ? hz : lz, bz = neg_z ? lz : hz;
+    const float t0 = fmaxf(fmaxf(fmaxf(tmin, ax), ay), az);
+    const float t1 = fminf(fminf(bx, by), bz);
+    return t0 <= tmax && t0 <= t1;
+}
+
+// The rays hit_box_signed serves (a traversal only ever lowers tmax to a hit distance, never NaN).
+WF_HD bool slab_signed_ok(const Vec3& o, const Vec3& inv_d, float tmin, float tmax) {
+    const float inf = __builtin_inff();
+    return fabsf(inv_d.x) < inf && fabsf(inv_d.y) < inf && fabsf(inv_d.z) < inf && inv_d.x != 0.0f &&
+           inv_d.y != 0.0f && inv_d.z != 0.0f && fabsf(o.x) < inf && fabsf(o.y) < inf && fabsf(o.z) < inf &&
+           tmin == tmin && tmax == tmax;
+}
+
 WF_HD bool prim_is_sphere(const SceneView& s, const Prim& p) { return p.id < s.n_spheres; }
 
 WF_HD bool hit_prim(const SceneView& s, const Prim& p, const Ray& r, float* t) {

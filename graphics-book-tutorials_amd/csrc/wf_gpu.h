@@ -220,29 +220,7 @@ __device__ __forceinline__ void trav_leaf(const Src& src, Traversal& tr, int& le
     }
 }
 
-// Slab test for a ray whose 1/d components are all finite, equal to hit_box bit for bit:
-//  - no slab bound is NaN: with a finite origin and 1/d finite and nonzero, b - o is finite or
-//    infinite (a box plane may be infinite) and its product with 1/d is never 0 * inf;
-//  - hit_box's swap tn > tf ? tf : tn takes the far plane's bound exactly when 1/d < 0: for
-//    1/d > 0, lo <= hi gives tn <= tf (subtracting o and multiplying by a positive number round
-//    monotonically); for 1/d < 0, tn >= tf, and where they are equal both choices have the same
-//    value up to the sign of a zero, which no comparison sees. neg_x/y/z are those signs, fixed
-//    per ray;
-//  - with no NaN, t0 <= fminf(fminf(fminf(tmax, bx), by), bz) is t0 <= tmax && t0 <= min(bx, by, bz).
-__device__ __forceinline__ bool hit_box_signed(const Vec3& lo, const Vec3& hi, const Vec3& o, const Vec3& inv_d,
-                                               float tmin, float tmax, bool neg_x, bool neg_y, bool neg_z) {
-    const float lx = (lo.x - o.x) * inv_d.x, hx = (hi.x - o.x) * inv_d.x;
-    const float ly = (lo.y - o.y) * inv_d.y, hy = (hi.y - o.y) * inv_d.y;
-    const float lz = (lo.z - o.z) * inv_d.z, hz = (hi.z - o.z) * inv_d.z;
-    const float ax = neg_x ? hx : lx, bx = neg_x ? lx : hx;
-    const float ay = neg_y ? hy : ly, by = neg_y ? ly : hy;
-    const float az = neg_z ? hz : lz, bz = neg_z ? lz : hz;
-    const float t0 = fmaxf(fmaxf(fmaxf(tmin, ax), ay), az);
-    const float t1 = fminf(fminf(bx, by), bz);
-    return t0 <= tmax && t0 <= t1;
-}
-
-// trav_visit for rays with finite 1/d (hit_box_signed). Device trees are checked when they are
+// trav_visit for the rays of slab_signed_ok (hit_box_signed). Device trees are checked when they are
 // laid out (every skip is -1 or a node, every child a node or a leaf code), so the next node needs
 // no range check: -1 ends the traversal. ALL_LDS: every node is in LDS.
 template <bool ALL_LDS, class Src>
@@ -279,19 +257,14 @@ __device__ __forceinline__ void wave_traverse_loop(int n_nodes, const Src& src, 
 // done). Each round either visits one node in every lane that has no parked leaf, or, once at
 // least `leaf_batch` lanes (or every live lane) have parked a leaf, tests those leaves together.
 // The rays do not change inside, so the visit form is chosen once per call for the whole wave:
-// the signed slab test unless a live ray has a 1/d component that is infinite, zero or NaN, or a
-// non-finite origin.
+// the signed slab test unless a live ray fails slab_signed_ok.
 template <class Src>
 __device__ __forceinline__ void wave_traverse(int n_nodes, const Src& src, Traversal& tr, int& leaf, bool anyhit,
                                               int limit, int leaf_batch) {
     limit = __builtin_amdgcn_readfirstlane(limit);
     leaf_batch = __builtin_amdgcn_readfirstlane(leaf_batch);
     const bool live = tr.node >= 0 || leaf >= 0;
-    const float inf = __builtin_inff();
-    const Vec3& v = tr.inv_d;
-    const Vec3& o = tr.r.o;
-    const bool finite = fabsf(v.x) < inf && fabsf(v.y) < inf && fabsf(v.z) < inf && v.x != 0.0f && v.y != 0.0f &&
-                        v.z != 0.0f && fabsf(o.x) < inf && fabsf(o.y) < inf && fabsf(o.z) < inf;
+    const bool finite = slab_signed_ok(tr.r.o, tr.inv_d, tr.r.tmin, tr.r.tmax);
     const bool neg_x = tr.inv_d.x < 0.0f, neg_y = tr.inv_d.y < 0.0f, neg_z = tr.inv_d.z < 0.0f;
     if (__ballot(live && !finite) != 0)
         wave_traverse_loop<0>(n_nodes, src, tr, leaf, anyhit, limit, leaf_batch, neg_x, neg_y, neg_z);

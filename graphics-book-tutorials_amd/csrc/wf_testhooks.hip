@@ -1,5 +1,6 @@
 // Estimator test hooks (include/wfamd.h): the same shared functions evaluated on the host and on
 // gfx950, so tests can show bit-for-bit agreement of the two compilations.
+#include <algorithm>
 #include <cstring>
 #include <stdexcept>
 #include <vector>
@@ -53,17 +54,63 @@ __global__ void sincos_grid_kernel(uint32_t k0, uint32_t k1, uint32_t per_block,
     if (threadIdx.x == 0) out[blockIdx.x] = h;
 }
 
-__global__ void intersect_kernel(SceneView s, const float* rays, int n, int anyhit, float* t, int* prim) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float* q = rays + 8 * i;
+// Ray of an 8-float record (o xyz, tmin, d xyz, tmax).
+__device__ __host__ inline Ray ray_of(const float* q) {
     Ray r;
     r.o = Vec3{q[0], q[1], q[2]};
     r.tmin = q[3];
     r.d = Vec3{q[4], q[5], q[6]};
     r.tmax = q[7];
+    return r;
+}
+
+// One slab case of 14 floats (lo xyz, hi xyz, o xyz, d xyz, tmin, tmax): out[0] = hit_box,
+// out[1] = hit_box_signed where slab_signed_ok holds, else -1.
+__device__ __host__ inline void slab_case(const float* q, int* out) {
+    const Vec3 lo{q[0], q[1], q[2]}, hi{q[3], q[4], q[5]}, o{q[6], q[7], q[8]}, d{q[9], q[10], q[11]};
+    const float tmin = q[12], tmax = q[13];
+    const Vec3 inv{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    out[0] = hit_box(lo, hi, o, inv, tmin, tmax) ? 1 : 0;
+    out[1] = slab_signed_ok(o, inv, tmin, tmax)
+                 ? (hit_box_signed(lo, hi, o, inv, tmin, tmax, inv.x < 0.0f, inv.y < 0.0f, inv.z < 0.0f) ? 1 : 0)
+                 : -1;
+}
+
+__global__ void slab_kernel(const float* q, int n, int* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) slab_case(q + 14 * i, out + 2 * i);
+}
+
+// The engines' wave-scheduled traversal (slice feed, wave_traverse with the signed, mixed and
+// exact visit forms) over the first n_lds nodes in LDS, and n_lds_prims primitive records.
+constexpr int kWaveTestThreads = 512;
+__global__ __launch_bounds__(kWaveTestThreads) void wave_intersect_kernel(SceneView s, const float* rays, int n,
+                                                                          int anyhit, float* t, int* prim, int n_lds,
+                                                                          int n_lds_prims) {
+    extern __shared__ Node lds[];
+    if (block_idle((uint32_t)n, 64)) return;
+    for (int k = threadIdx.x; k < n_lds; k += blockDim.x) lds[k] = s.nodes[k];
+    float4* lprim = reinterpret_cast<float4*>(lds + n_lds);
+    for (int k = threadIdx.x; k < n_lds_prims; k += blockDim.x) lprim[k] = lds_prim_record(s, k);
+    __syncthreads();
+    const LdsNodes src = lds_nodes_of(s, lds, n_lds, n_lds_prims > 0 ? lprim : nullptr);
+    trace_items(
+        (uint32_t)n, 64, s.n_nodes, src, anyhit != 0, 32, 16,
+        [&](uint32_t i, uint32_t* tag) {
+            *tag = i;
+            return ray_of(rays + 8 * i);
+        },
+        [&](uint32_t i, uint32_t, const Traversal& tr) {
+            t[i] = tr.r.tmax;
+            prim[i] = tr.prim >= 0 ? s.prims[tr.prim].id : -1;
+        });
+}
+
+__global__ void intersect_kernel(SceneView s, const float* rays, int n, int anyhit, float* t, int* prim) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
     Hit h;
-    intersect(s, r, &h, anyhit != 0);
+    intersect(s, ray_of(rays + 8 * i), &h, anyhit != 0);
     t[i] = h.t;
     prim[i] = h.prim >= 0 ? s.prims[h.prim].id : -1;
 }
@@ -181,21 +228,33 @@ int wf_test_sincos_grid_hash(uint32_t k0, uint32_t k1, int device, uint64_t* has
     });
 }
 
+int wf_test_slab(const float* cases, int n, int* out, int device) {
+    return hook([&] {
+        if (n < 0 || (n > 0 && (!cases || !out))) throw std::invalid_argument("bad argument");
+        if (!device) {
+            for (int i = 0; i < n; ++i) slab_case(cases + 14 * i, out + 2 * i);
+            return;
+        }
+        DevBuf<float> dq((size_t)14 * n);
+        DevBuf<int> dout((size_t)2 * n);
+        dq.put(cases, (size_t)14 * n);
+        if (n) slab_kernel<<<(n + 255) / 256, 256>>>(dq.p, n, dout.p);
+        check(hipGetLastError(), "slab kernel");
+        check(hipDeviceSynchronize(), "slab sync");
+        dout.get(out, (size_t)2 * n);
+    });
+}
+
 int wf_test_intersect(const wf_scene* scene, const float* rays, int n, int anyhit, float* t, int* prim, int device) {
     return hook([&] {
         if (!scene || n < 0 || (n > 0 && (!rays || !t || !prim))) throw std::invalid_argument("bad argument");
+        if (device < 0 || device > 3) throw std::invalid_argument("device must be 0..3");
         const Scene& sc = scene->s;
         if (!device) {
             const SceneView v = host_view(sc);
             for (int i = 0; i < n; ++i) {
-                const float* q = rays + 8 * i;
-                Ray r;
-                r.o = Vec3{q[0], q[1], q[2]};
-                r.tmin = q[3];
-                r.d = Vec3{q[4], q[5], q[6]};
-                r.tmax = q[7];
                 Hit h;
-                intersect(v, r, &h, anyhit != 0);
+                intersect(v, ray_of(rays + 8 * i), &h, anyhit != 0);
                 t[i] = h.t;
                 prim[i] = h.prim >= 0 ? sc.dprims[h.prim].id : -1;
             }
@@ -206,7 +265,17 @@ int wf_test_intersect(const wf_scene* scene, const float* rays, int n, int anyhi
         DevBuf<float> dr((size_t)8 * n), dt(n);
         DevBuf<int> dp(n);
         dr.put(rays, (size_t)8 * n);
-        if (n) intersect_kernel<<<(n + 255) / 256, 256>>>(ds.view, dr.p, n, anyhit, dt.p, dp.p);
+        if (n && device == 1) intersect_kernel<<<(n + 255) / 256, 256>>>(ds.view, dr.p, n, anyhit, dt.p, dp.p);
+        if (n && device >= 2) {
+            const int nn = ds.view.n_nodes, np = ds.view.n_prims;
+            const int n_lds = device == 2 ? std::min(nn, 1024) : std::min(nn, 8);
+            const int n_lds_prims = (device == 2 && n_lds == nn && (size_t)np * 16 + (size_t)n_lds * 32 <= 64 * 1024) ? np : 0;
+            const size_t lds = (size_t)n_lds * sizeof(Node) + (size_t)n_lds_prims * 16;
+            const int per_block = 64 * (kWaveTestThreads / 64);
+            const int blocks = std::min((n + per_block - 1) / per_block, 1024);
+            wave_intersect_kernel<<<blocks, kWaveTestThreads, lds>>>(ds.view, dr.p, n, anyhit, dt.p, dp.p, n_lds,
+                                                                     n_lds_prims);
+        }
         check(hipGetLastError(), "intersect kernel");
         check(hipDeviceSynchronize(), "intersect sync");
         dt.get(t, n);

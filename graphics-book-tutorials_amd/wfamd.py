@@ -118,7 +118,7 @@ EXPORTS = [
     "wf_scan_workspace_destroy", "wf_scan_exclusive_u32", "wf_compact_u32", "wf_test_queue_push",
     "wf_batch_render", "wf_autotune", "wf_server_create", "wf_server_destroy", "wf_server_submit",
     "wf_server_step", "wf_server_poll", "wf_server_stats_get", "wf_test_sincos", "wf_test_sincos_grid_hash",
-    "wf_test_intersect", "wf_test_trace_paths", "wf_copy_probe",
+    "wf_test_intersect", "wf_test_slab", "wf_test_trace_paths", "wf_copy_probe",
 ]
 
 
@@ -168,6 +168,7 @@ def lib():
         "wf_test_sincos": (I, [F, I, F, F, I]),
         "wf_test_sincos_grid_hash": (I, [C.c_uint32, C.c_uint32, I, C.POINTER(C.c_uint64)]),
         "wf_test_intersect": (I, [P, F, I, I, F, C.POINTER(C.c_int), I]),
+        "wf_test_slab": (I, [F, I, C.POINTER(C.c_int), I]),
         "wf_test_trace_paths": (I, [P, C.POINTER(C.c_int), I, C.c_uint32, I, F, C.POINTER(C.c_uint32), I]),
     }
     for name, (res, args) in sig.items():
@@ -404,7 +405,17 @@ def test_sincos_grid_hash(k0=0, k1=1 << 24, device=False):
     return h.value
 
 
+def test_slab(cases, device=False):
+    """(reference slab test, signed slab test or -1) per 14-float case."""
+    cases = np.ascontiguousarray(cases, np.float32).reshape(-1, 14)
+    out = np.zeros((len(cases), 2), np.int32)
+    _check(lib().wf_test_slab(_fptr(cases), len(cases), _iptr(out), int(device)))
+    return out
+
+
 def test_intersect(scene, rays, anyhit=False, device=False):
+    """device: False/0 host, True/1 one GPU thread per ray, 2 wave-scheduled traversal with the
+    tree top in LDS, 3 the same with 8 nodes in LDS."""
     rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
     t = np.zeros(len(rays), np.float32)
     p = np.zeros(len(rays), np.int32)

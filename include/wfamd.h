@@ -190,8 +190,16 @@ int wf_test_sincos(const float* x, int n, float* s, float* c, int device);
  * stream, so host and device can be compared over the whole 2^24 grid without moving 128 MiB. */
 int wf_test_sincos_grid_hash(uint32_t k0, uint32_t k1, int device, uint64_t* hash);
 /* Closest (anyhit = 0) or any hit for n rays (8 floats: o xyz, tmin, d xyz, tmax); returns the
- * hit distance and the primitive id in the reference's id space (-1 for a miss). */
+ * hit distance and the primitive id in the reference's id space (-1 for a miss). device: 0 host,
+ * 1 one GPU thread per ray, 2 the engines' wave-scheduled traversal with the tree top (up to 1024
+ * nodes, and the primitive records when the whole tree fits) in LDS, 3 the same with only 8 nodes
+ * in LDS (the rest read from memory). */
 int wf_test_intersect(const wf_scene* scene, const float* rays, int n, int anyhit, float* t, int* prim, int device);
+/* Slab tests of n cases (14 floats: box lo xyz, hi xyz, ray o xyz, d xyz, tmin, tmax), 1/d taken
+ * as the traversal does: out[2i] = the reference-order slab test (include/wf/bvh.hpp:64-78),
+ * out[2i+1] = the per-ray-signed form the GPU traversal uses, or -1 where that form does not apply
+ * (an infinite or zero 1/d component, a non-finite origin, a NaN tmin or tmax). */
+int wf_test_slab(const float* cases, int n, int* out, int device);
 /* Radiance of single paths (px, py, sample triples) with the shared estimator, and per-path ray
  * counts (extension, shadow). */
 int wf_test_trace_paths(const wf_scene* scene, const int* pxys, int n, uint32_t seed, int max_depth, float* L,

@@ -8,7 +8,10 @@ Fixtures (data only):
                   and of prim_order (src/scenes.cpp, src/bvh_build.cpp)
   ref_render.npz  render_cpu mean images of the reference scenes at 16x9, 4 spp, seed 1234, depth 8
   ref_paths.npz   trace_path radiance of 64 single paths per reference scene
-  intersect.npz   closest-hit (t, prim) of 1024 seeded rays per reference scene
+  intersect.npz   closest-hit (t, prim) of 1024 seeded rays per reference scene; and closest- and
+                  any-hit results of 2048 axis-aligned rays (two exact zeros in d, of either sign,
+                  so 1/d has infinite components) from origins on BVH node planes, the cases where
+                  the slab test meets 0 * inf
 """
 import ctypes as C
 import json
@@ -38,6 +41,27 @@ def seeded_rays(n, seed, center, radius):
     o = center + rs.uniform(-radius, radius, size=(n, 3))
     d = rs.normal(size=(n, 3))
     d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 3] = 1e-4
+    rays[:, 4:7] = d
+    rays[:, 7] = 3.402823466e38
+    return rays
+
+
+def axis_rays(nodes, n, seed):
+    """Rays along an axis (the other two components +0 or -0) from origins on node box planes
+    (sometimes nudged by 1e-3), tmin 1e-4, tmax FLT_MAX. nodes: int32 rows with lo at columns 0-2
+    and hi at 4-6 (float bits)."""
+    lo = np.ascontiguousarray(nodes[:, 0:3]).view(np.float32)
+    hi = np.ascontiguousarray(nodes[:, 4:7]).view(np.float32)
+    rs = np.random.default_rng(seed)
+    k = rs.integers(0, len(nodes), n)
+    o = np.where(rs.random((n, 3)) < 0.5, lo[k], hi[k]) + \
+        rs.choice(np.array([0.0, 0.0, 1e-3, -1e-3], np.float32), (n, 3))
+    d = np.zeros((n, 3), np.float32)
+    d[np.arange(n), rs.integers(0, 3, n)] = rs.choice([-1.0, 1.0], n)
+    d[(d == 0) & (rs.random((n, 3)) < 0.5)] = -0.0
     rays = np.zeros((n, 8), np.float32)
     rays[:, 0:3] = o
     rays[:, 3] = 1e-4
@@ -94,6 +118,14 @@ def main():
         inter[name + "_rays"] = rays
         inter[name + "_t"] = t
         inter[name + "_prim"] = p
+        ar = axis_rays(nodes, 2048, 17)
+        t, p = s.intersect(ar, False)
+        ta, pa = s.intersect(ar, True)
+        inter[name + "_axis_rays"] = ar
+        inter[name + "_axis_t"] = t
+        inter[name + "_axis_prim"] = p
+        inter[name + "_axis_any_t"] = ta
+        inter[name + "_axis_any_prim"] = pa
     with open(os.path.join(OUT, "scenes.json"), "w") as f:
         json.dump(scenes, f, indent=1)
     np.savez_compressed(os.path.join(OUT, "ref_render.npz"), **renders)
