@@ -58,7 +58,7 @@ struct PersistentTuning {
     int block_size = 0;    // threads per workgroup: 256, 512 or 1024 (0 -> 512)
     int pool_paths = 0;    // path slots = queue capacity (0 -> 2^22)
     int lds_kib = 0;       // LDS per workgroup for BVH nodes and primitive records (0 -> 64 KiB; at most the
-                           // device limit less 1 KiB, 159 KiB on gfx950)
+                           // device limit less the kernels' own LDS, under 1 KiB)
     int graph_iters = 0;   // iterations per captured graph (0 -> 16; rounded up to even)
     int refill = 0;        // traversal refill threshold in lanes of 64 (0 -> 32)
     int leaf_batch = 0;    // parked leaves that trigger a wave's leaf-test round (0 -> 16)

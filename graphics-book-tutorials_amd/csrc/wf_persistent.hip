@@ -568,7 +568,8 @@ struct PersistentEngine::Impl {
     hipStream_t stream = nullptr;
     DevCounters* ctr = nullptr;
     int block = kDefaultBlock, graph_iters = kDefaultGraphIters, lds_bytes = kDefaultLdsKiB * 1024, cus = 0;
-    int lds_max = 0;   // LDS a workgroup may use on this device (bytes)
+    int lds_max = 0;      // LDS a workgroup may use on this device (bytes)
+    int lds_static = 0;   // the traversal kernels' own LDS (bytes)
     int refill = kDefaultRefill;
     int leaf_batch = kDefaultLeafBatch;
     int dir_bins = kDefaultDirBins;
@@ -659,6 +660,12 @@ void PersistentEngine::ensure_pool(const PersistentTuning& t) {
         check(hipDeviceGetAttribute(&m.cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
         check(hipDeviceGetAttribute(&m.lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev),
               "hipDeviceGetAttribute");
+        for (const void* fn : {(const void*)pw_extend_kernel<true>, (const void*)pw_extend_kernel<false>,
+                               (const void*)pw_shadow_kernel<true>, (const void*)pw_shadow_kernel<false>}) {
+            hipFuncAttributes a{};
+            check(hipFuncGetAttributes(&a, fn), "hipFuncGetAttributes");
+            m.lds_static = std::max(m.lds_static, (int)a.sharedSizeBytes);
+        }
     }
     int cap = t.pool_paths > 0 ? t.pool_paths : kDefaultPool;
     cap = (cap + 1023) & ~1023;
@@ -679,8 +686,9 @@ void PersistentEngine::ensure_pool(const PersistentTuning& t) {
     m.ctx.dir_bins = m.dir_bins;
     m.graph_iters = t.graph_iters > 0 ? ((t.graph_iters + 1) & ~1) : kDefaultGraphIters;
     const int kib = t.lds_kib > 0 ? t.lds_kib : kDefaultLdsKiB;
-    // The kernels' own LDS (the plan and the shade round counters) stays below 1 KiB.
-    if (kib * 1024 > m.lds_max - 1024)
+    // The budget is the dynamic LDS; the kernels' own (the plan and the segment tables)
+    // comes on top.
+    if (kib * 1024 > m.lds_max - m.lds_static)
         throw std::runtime_error("persistent: LDS budget above what a workgroup may use on this device");
     if (m.lds_bytes != kib * 1024) {
         m.lds_bytes = kib * 1024;
