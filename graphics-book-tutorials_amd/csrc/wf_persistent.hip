@@ -1139,8 +1139,8 @@ PersistentTuning autotune_persistent(const PersistentJob* jobs, int n_jobs, hipS
     // Stage 2: traversal scheduling (refill threshold x leaf batch) for that workgroup size, then
     // survivor grouping.
     PersistentTuning base = best;
-    for (int rf : {8, 16, 32})
-        for (int lb : {4, 8, 16, 32}) {
+    for (int rf : {16, 32, 48})
+        for (int lb : {8, 16, 24, 32}) {
             if (rf == base.refill && lb == base.leaf_batch) continue;
             PersistentTuning t = base;
             t.refill = rf;
