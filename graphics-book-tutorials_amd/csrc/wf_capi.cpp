@@ -693,8 +693,10 @@ extern "C" int wf_batch_render(const wf_job* jobs, int n_jobs, const wf_render_p
         wf::gpu::check(hipEventCreate(&e0), "hipEventCreate");
         wf::gpu::check(hipEventCreate(&e1), "hipEventCreate");
         wf::gpu::PersistentResult pr;
+        wf::StageTimes stages;
+        const bool timed = tuning && tuning->stage_timing;
         wf::gpu::check(hipEventRecord(e0, b.stream), "hipEventRecord");
-        eng.render(b.jobs.data(), n_jobs, t, b.ctr, b.stream, &pr);
+        eng.render(b.jobs.data(), n_jobs, t, b.ctr, b.stream, &pr, timed ? &stages : nullptr);
         wf::gpu::check(hipEventRecord(e1, b.stream), "hipEventRecord");
         wf::gpu::check(hipEventSynchronize(e1), "batch sync");
         float ms = 0.0f;
@@ -721,6 +723,14 @@ extern "C" int wf_batch_render(const wf_job* jobs, int n_jobs, const wf_render_p
             stats->dir_bins = t.dir_bins;
             stats->shade_block = t.shade_block;
             stats->device_bytes = eng.bytes() + b.scenes.bytes();
+            const double ms8[8] = {stages.extend,           stages.classify, stages.shade_lambert, stages.shade_metal,
+                                   stages.shade_dielectric, stages.shadow,   stages.compact,       stages.other};
+            for (int k = 0; k < 8; ++k) {
+                stats->stage_ms[k] = ms8[k];
+                stats->stage_items[k] = stages.items[k];
+                stats->stage_launches[k] = stages.launches[k];
+                stats->stage_bytes[k] = stages.bytes[k];
+            }
         }
         return (int)WF_OK;
     });

@@ -155,9 +155,7 @@ public:
     void run_profiled(StageTimes* out);
     PersistentProgress progress();
     bool job_complete(int job) const;
-    // keep_buffer: keep the sample buffer for the next admitted job of a similar size instead of
-    // freeing it (render() does; a caller that budgets device memory itself does not).
-    void finish_job(int job, bool keep_buffer = false);
+    void finish_job(int job);
     void reset_jobs();
     uint64_t pool_bytes() const { return pool_bytes_; }
     uint64_t bytes() const;
@@ -172,6 +170,8 @@ private:
     void ensure_pool(const PersistentTuning& t);
     void capture(bool single);
     void update_lds_plan();
+    JobDev prepare_job(const PersistentJob& job, float4* sbuf);
+    void admit_chunk(const std::vector<PersistentJob>& chunk);
     uint64_t sbuf_budget_ = 4ull << 30;
     int cap_ = 0;
     uint64_t pool_bytes_ = 0;

@@ -57,14 +57,57 @@ __device__ __forceinline__ const JobDev& job_of(const PCtx& c, uint32_t jid) {
     else return c.jobs[jid];
 }
 
-__device__ __forceinline__ int find_job(const JobDev* jobs, uint32_t n_jobs, unsigned long long w) {
+// First job whose work range ends after w; end(k) is job k's work_end (increasing).
+template <class End>
+__device__ __forceinline__ int find_job(End end, uint32_t n_jobs, unsigned long long w) {
     int lo = 0, hi = (int)n_jobs - 1;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (jobs[mid].work_end > w) hi = mid;
+        if (end(mid) > w) hi = mid;
         else lo = mid + 1;
     }
     return lo;
+}
+
+// Multi-job kernels stage the scene table and, for batches of up to kLdsJobs jobs, each job's work
+// end, scene and depth limit in LDS. Otherwise a lane reaches its scene through two dependent
+// memory loads (job record, then scene record), and a new path finds its job by a binary search
+// over the job records in memory: chains of latency before any work starts.
+constexpr int kLdsJobs = 256;
+struct JobTab {
+    SceneView scenes[kMaxScenes];
+    unsigned long long work_end[kLdsJobs];
+    int scene[kLdsJobs];
+    int max_depth[kLdsJobs];
+    int staged;   // the job arrays hold every job of the batch
+};
+
+// The whole block calls it. done (optional): the workgroup's finished-path counters, cleared.
+__device__ __forceinline__ void stage_jobs(const PCtx& c, uint32_t n_jobs, JobTab* t, uint32_t* done = nullptr) {
+    if (done)
+        for (int i = threadIdx.x; i < kLdsJobs; i += blockDim.x) done[i] = 0u;
+    const int n_scenes = c.plan->n;
+    const int* src = reinterpret_cast<const int*>(c.scenes);
+    int* dst = reinterpret_cast<int*>(t->scenes);
+    const int words = n_scenes * (int)(sizeof(SceneView) / sizeof(int));
+    for (int i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
+    const bool staged = n_jobs <= (uint32_t)kLdsJobs;
+    if (staged)
+        for (int i = threadIdx.x; i < (int)n_jobs; i += blockDim.x) {
+            t->work_end[i] = c.jobs[i].work_end;
+            t->scene[i] = c.jobs[i].scene;
+            t->max_depth[i] = c.jobs[i].max_depth;
+        }
+    if (threadIdx.x == 0) t->staged = staged ? 1 : 0;
+    __syncthreads();
+}
+
+__device__ __forceinline__ int job_scene(const PCtx& c, const JobTab* t, uint32_t jid) {
+    return t->staged ? t->scene[jid] : c.jobs[jid].scene;
+}
+
+__device__ __forceinline__ int job_depth(const PCtx& c, const JobTab* t, uint32_t jid) {
+    return t->staged ? t->max_depth[jid] : c.jobs[jid].max_depth;
 }
 
 // Pixel of position q in a pass, visiting the image in 8-row bands of 8x8 tiles (edge tiles
@@ -81,12 +124,15 @@ __device__ __forceinline__ void tile_pixel(uint32_t q, uint32_t W, uint32_t H, u
 }
 
 // The path of work index w: its job, its sample-buffer index (sample-major, then pixel) and its
-// initial state (include/wf/path_core.hpp:59-82).
+// initial state (include/wf/path_core.hpp:59-82). t: the multi-job kernels' LDS tables.
 template <bool SINGLE>
-__device__ __forceinline__ PathState new_path(const PCtx& c, uint32_t n_jobs, unsigned long long w, uint32_t* jid_out,
-                                              uint32_t* sidx_out) {
+__device__ __forceinline__ PathState new_path(const PCtx& c, const JobTab* t, uint32_t n_jobs, unsigned long long w,
+                                              uint32_t* jid_out, uint32_t* sidx_out) {
     uint32_t jid = 0;
-    if constexpr (!SINGLE) jid = (uint32_t)find_job(c.jobs, n_jobs, w);
+    if constexpr (!SINGLE) {
+        if (t->staged) jid = (uint32_t)find_job([&](int k) { return t->work_end[k]; }, n_jobs, w);
+        else jid = (uint32_t)find_job([&](int k) { return c.jobs[k].work_end; }, n_jobs, w);
+    }
     const JobDev& job = job_of<SINGLE>(c, jid);
     const uint32_t local = (uint32_t)(w - job.work_begin);
     const uint32_t W = (uint32_t)job.width, WH = W * (uint32_t)job.height;
@@ -131,7 +177,10 @@ __device__ __forceinline__ LdsNodes src_of(const SceneView& s, const Node* lds, 
                         sp->prim_cnt[sc] > 0 ? lprim + sp->prim_off[sc] : nullptr);
 }
 
-// Adds one per lane with pred to job_done[jid], one atomic per distinct job id in the wave.
+// Adds one per lane with pred to job_done[jid], one atomic per distinct job id in the wave. The
+// multi-job kernels pass their workgroup's LDS counters when the batch's jobs are staged, and add
+// them to the device counters once per job at the end (flush_done): waves of the whole grid would
+// otherwise serialise on the counters of the few jobs in flight.
 __device__ __forceinline__ void count_done(uint32_t* job_done, bool pred, uint32_t jid) {
     uint64_t m = __ballot(pred);
     while (m) {
@@ -141,6 +190,13 @@ __device__ __forceinline__ void count_done(uint32_t* job_done, bool pred, uint32
         if ((int)lane_id() == leader) atomicAdd(&job_done[j0], (uint32_t)__popcll(same));
         m &= ~same;
     }
+}
+
+// Adds the workgroup's LDS counters to the device counters; the whole block calls it at its end.
+__device__ __forceinline__ void flush_done(const uint32_t* done, uint32_t* job_done, uint32_t n_jobs) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < (int)n_jobs; i += blockDim.x)
+        if (done[i]) atomicAdd(&job_done[i], done[i]);
 }
 
 // Output window of each segment for a shade pass over n items in rounds of B: segment s holds the
@@ -167,6 +223,17 @@ __device__ __forceinline__ uint32_t seg_pos(const uint32_t* prefix, const uint32
 __global__ void pw_init_kernel(PState* st) {
     PState z{};
     *st = z;
+}
+
+// Jobs [first, first + n) have had their records copied in; total: the work end of the last.
+__global__ void pw_admit_many_kernel(PState* st, uint32_t* job_done, uint32_t first, uint32_t n,
+                                     unsigned long long total) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) job_done[first + i] = 0u;
+    if (threadIdx.x == 0) {
+        st->n_jobs = first + n;
+        st->total_work = total;
+        st->done = 0;
+    }
 }
 
 __global__ void pw_admit_kernel(PState* st, JobDev* jobs, uint32_t* job_done, JobDev job, uint32_t index) {
@@ -222,6 +289,7 @@ template <bool SINGLE>
 __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
     extern __shared__ Node lds[];
     __shared__ LdsPlan sp;
+    __shared__ JobTab jt;   // multi-job only
     const PState* st = c.st;
     const uint32_t n_act = st->it_act, n_live = st->it_live, n_jobs = st->n_jobs;
     const unsigned long long wbase = st->it_work_base;
@@ -229,6 +297,7 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
     __shared__ uint32_t pre[kSegs + 1], bas[kSegs];
     if (threadIdx.x <= kSegs) pre[threadIdx.x] = st->in_prefix[threadIdx.x];
     if (threadIdx.x < kSegs) bas[threadIdx.x] = st->in_base[threadIdx.x];
+    if constexpr (!SINGLE) stage_jobs(c, n_jobs, &jt);
     stage_lds<SINGLE>(c, lds, &sp);   // synchronises
     WaveFeed f;
     feed_first(f, n_act);
@@ -258,15 +327,15 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
                     r.d = Vec3{d.x, d.y, d.z};
                     r.tmin = kRayEps;
                     r.tmax = kInf;
-                    if constexpr (!SINGLE) sc = c.jobs[in.kw[p].z].scene;
+                    if constexpr (!SINGLE) sc = job_scene(c, &jt, in.kw[p].z);
                 } else {
                     uint32_t jid, sidx;
-                    r = new_path<SINGLE>(c, n_jobs, wbase + (item - n_live), &jid, &sidx).ray;
-                    if constexpr (!SINGLE) sc = c.jobs[jid].scene;
+                    r = new_path<SINGLE>(c, &jt, n_jobs, wbase + (item - n_live), &jid, &sidx).ray;
+                    if constexpr (!SINGLE) sc = job_scene(c, &jt, jid);
                 }
                 if constexpr (!SINGLE) {
-                    src = src_of(c.scenes[sc], lds, &sp, sc);
-                    nn = c.scenes[sc].n_nodes;
+                    src = src_of(jt.scenes[sc], lds, &sp, sc);
+                    nn = jt.scenes[sc].n_nodes;
                 }
                 tr = traversal_begin(r);
                 if (nn <= 0) tr.node = -1;
@@ -280,7 +349,10 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
         wave_traverse(nn, src, tr, leaf, false, (f.dry && f.cursor >= f.end) ? 1 : c.refill, c.leaf_batch);
         if (item != kIdle && tr.node < 0 && leaf < 0) {
             Hit h{tr.r.tmax, tr.prim};
-            if (!hit_is_valid(scene_of<SINGLE>(c, sc), h)) {
+            const SceneView* hs;
+            if constexpr (SINGLE) hs = &c.scene0;
+            else hs = &jt.scenes[sc];
+            if (!hit_is_valid(*hs, h)) {
                 inv += 1;
                 h.prim = -1;
             }
@@ -321,12 +393,15 @@ __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, Path
     __shared__ uint32_t cnt[2][10][16];   // per round parity: 8 direction bins, shadow, finished
     __shared__ uint32_t bases[2][2];
     __shared__ uint32_t pre[kSegs + 1], bas[kSegs], win[kSegs];
+    __shared__ JobTab jt;                   // multi-job only
+    __shared__ uint32_t done_cnt[kLdsJobs];   // multi-job only
     PState* st = c.st;
     const uint32_t n = st->it_act, n_live = st->it_live, n_jobs = st->n_jobs;
     const unsigned long long wbase = st->it_work_base;
     const uint32_t wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
     const uint32_t stride = gridDim.x * blockDim.x;
     if (blockIdx.x * blockDim.x >= n) return;   // no round for this workgroup
+    if constexpr (!SINGLE) stage_jobs(c, n_jobs, &jt, done_cnt);
     if (threadIdx.x <= kSegs) pre[threadIdx.x] = st->in_prefix[threadIdx.x];
     if (threadIdx.x < kSegs) bas[threadIdx.x] = st->in_base[threadIdx.x];
     if (threadIdx.x == 0) seg_windows(n, blockDim.x, win);
@@ -365,14 +440,23 @@ __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, Path
                 jid = kw.z;
                 sidx = kw.w;
             } else {
-                ps = new_path<SINGLE>(c, n_jobs, wbase + (item - n_live), &jid, &sidx);
+                ps = new_path<SINGLE>(c, &jt, n_jobs, wbase + (item - n_live), &jid, &sidx);
             }
             key = ps.key;
             const JobDev& job = job_of<SINGLE>(c, jid);
-            const SceneView& s = scene_of<SINGLE>(c, job.scene);
+            const SceneView* sv;
+            int depth;
+            if constexpr (SINGLE) {
+                sv = &c.scene0;
+                depth = job.max_depth;
+            } else {
+                sv = &jt.scenes[job_scene(c, &jt, jid)];
+                depth = job_depth(c, &jt, jid);
+            }
+            const SceneView& s = *sv;
             const Hit h{hv.x, __float_as_int(hv.y)};
             Surface f;
-            const int mat = classify(s, ps, h, job.max_depth, &f);
+            const int mat = classify(s, ps, h, depth, &f);
             if (mat == kLambert) live = shade_lambert(s, ps, f, &sh, &contrib, &has);
             else if (mat == kMetal) live = shade_metal(s, ps, f);
             else if (mat == kDielectric) live = shade_dielectric(s, ps, f);
@@ -431,8 +515,13 @@ __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, Path
             c.SQ.d[e] = make_float4(sh.d.x, sh.d.y, sh.d.z, __int_as_float((int)(jid | (live ? 0u : kFinalFlag))));
             c.SQ.c[e] = make_float4(contrib.x, contrib.y, contrib.z, __int_as_float((int)(live ? o : sidx)));
         }
-        if constexpr (!SINGLE) count_done(c.job_done, fin, jid);
+        if constexpr (!SINGLE) {
+            if (jt.staged) count_done(done_cnt, fin, jid);
+            else count_done(c.job_done, fin, jid);
+        }
     }
+    if constexpr (!SINGLE)
+        if (jt.staged) flush_done(done_cnt, c.job_done, n_jobs);
 }
 
 // Any-hit traversal of the shadow rays, "while-while" like extend. An unoccluded ray adds its
@@ -442,6 +531,8 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
     extern __shared__ Node lds[];
     __shared__ LdsPlan sp;
     __shared__ uint32_t pre[kSegs + 1], win[kSegs], shc[kSegs], svc[kSegs];
+    __shared__ JobTab jt;                   // multi-job only
+    __shared__ uint32_t done_cnt[kLdsJobs];   // multi-job only
     PState* st = c.st;
     if (threadIdx.x < kSegs) {
         shc[threadIdx.x] = st->sh_cnt[threadIdx.x];
@@ -469,6 +560,7 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
     __syncthreads();
     const uint32_t n = pre[kSegs];
     if (block_idle(n)) return;
+    if constexpr (!SINGLE) stage_jobs(c, st->n_jobs, &jt, done_cnt);
     stage_lds<SINGLE>(c, lds, &sp);
     WaveFeed f;
     feed_first(f, n);
@@ -499,9 +591,9 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
                 r.tmin = kRayEps;
                 r.tmax = o.w;
                 if constexpr (!SINGLE) {
-                    const int sc = c.jobs[tag & ~kFinalFlag].scene;
-                    src = src_of(c.scenes[sc], lds, &sp, sc);
-                    nn = c.scenes[sc].n_nodes;
+                    const int sc = job_scene(c, &jt, tag & ~kFinalFlag);
+                    src = src_of(jt.scenes[sc], lds, &sp, sc);
+                    nn = jt.scenes[sc].n_nodes;
                 }
                 tr = traversal_begin(r);
                 if (nn <= 0) tr.node = -1;
@@ -530,8 +622,13 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
             }
             item = kIdle;
         }
-        if constexpr (!SINGLE) count_done(c.job_done, fin, jid);
+        if constexpr (!SINGLE) {
+            if (jt.staged) count_done(done_cnt, fin, jid);
+            else count_done(c.job_done, fin, jid);
+        }
     }
+    if constexpr (!SINGLE)
+        if (jt.staged) flush_done(done_cnt, c.job_done, st->n_jobs);
 }
 
 // Adds a finished job's samples to its film, per pixel in sample order.
@@ -587,6 +684,7 @@ struct PersistentEngine::Impl {
         uint64_t work = 0;
         uint32_t done = 0;
         bool finished = false;
+        bool owns_buffer = true;   // false: a slice of the render arena
     };
     std::vector<HostJob> hjobs;
     std::vector<SceneView> hscenes;
@@ -610,9 +708,10 @@ struct PersistentEngine::Impl {
     } graph_key;
     uint64_t scene_gen = 1;   // bumped whenever the scene table changes
     uint64_t sbuf_bytes = 0;
-    // One finished job's sample buffer kept for the next job of a similar size (render() only).
-    float4* spare = nullptr;
-    uint64_t spare_bytes = 0;
+    // render(): one allocation holds the sample buffers of a chunk of jobs; kept across renders.
+    float4* arena = nullptr;
+    uint64_t arena_bytes = 0;
+    std::vector<JobDev> staged;   // the device records of the last chunk (source of its copy)
 
     void free_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -620,10 +719,10 @@ struct PersistentEngine::Impl {
         exec = nullptr;
         graph = nullptr;
     }
-    void free_spare() {
-        if (spare) (void)hipFree(spare);
-        spare = nullptr;
-        spare_bytes = 0;
+    void free_arena() {
+        if (arena) (void)hipFree(arena);
+        arena = nullptr;
+        arena_bytes = 0;
     }
     void free_pool() {
         for (void* p : pool_allocs) (void)hipFree(p);
@@ -631,9 +730,9 @@ struct PersistentEngine::Impl {
     }
     ~Impl() {
         free_graph();
-        free_spare();
         for (HostJob& j : hjobs)
-            if (j.dev.sbuf) (void)hipFree(j.dev.sbuf);
+            if (j.dev.sbuf && j.owns_buffer) (void)hipFree(j.dev.sbuf);
+        free_arena();
         free_pool();
         if (st_host) (void)hipHostFree(st_host);
         if (done_host) (void)hipHostFree(done_host);
@@ -650,7 +749,7 @@ struct PersistentEngine::LaunchPlan {
 PersistentEngine::PersistentEngine() : m_(std::make_unique<Impl>()) {}
 PersistentEngine::~PersistentEngine() = default;
 
-uint64_t PersistentEngine::bytes() const { return pool_bytes_ + m_->sbuf_bytes + m_->spare_bytes; }
+uint64_t PersistentEngine::bytes() const { return pool_bytes_ + m_->sbuf_bytes + m_->arena_bytes; }
 
 void PersistentEngine::ensure_pool(const PersistentTuning& t) {
     Impl& m = *m_;
@@ -754,7 +853,7 @@ void PersistentEngine::open(const PersistentTuning& t, DevCounters* ctr, hipStre
 void PersistentEngine::reset_jobs() {
     Impl& m = *m_;
     for (Impl::HostJob& j : m.hjobs)
-        if (j.dev.sbuf) (void)hipFree(j.dev.sbuf);
+        if (j.dev.sbuf && j.owns_buffer) (void)hipFree(j.dev.sbuf);
     m.hjobs.clear();
     m.sbuf_bytes = 0;
     m.total_work = 0;
@@ -802,7 +901,9 @@ void PersistentEngine::update_lds_plan() {
     check(hipMemcpyAsync(m.ctx.plan, &m.lds_plan, sizeof(LdsPlan), hipMemcpyHostToDevice, m.stream), "copy lds plan");
 }
 
-int PersistentEngine::admit(const PersistentJob& job) {
+// Validates a job, enters its scene in the scene table and returns its device record, its work
+// range following the jobs admitted so far; sbuf: its sample buffer, or null to allocate one.
+JobDev PersistentEngine::prepare_job(const PersistentJob& job, float4* sbuf) {
     Impl& m = *m_;
     if ((int)m.hjobs.size() >= kMaxJobs) throw std::runtime_error("persistent: job table full");
     if (job.spp < 1 || job.scene == nullptr) throw std::runtime_error("persistent: invalid job");
@@ -843,25 +944,53 @@ int PersistentEngine::admit(const PersistentJob& job) {
     d.work_begin = m.total_work;
     d.work_end = m.total_work + work;
     d.film = job.film;
-    const uint64_t need = (uint64_t)work * sizeof(float4);
-    if (m.spare && m.spare_bytes >= need && m.spare_bytes <= 2 * need) {
-        d.sbuf = m.spare;
-        m.spare = nullptr;
-        m.spare_bytes = 0;
-    } else {
-        m.free_spare();
-        check(hipMalloc(&d.sbuf, (size_t)need), "hipMalloc sample buffer");
-    }
-    m.sbuf_bytes += (uint64_t)work * sizeof(float4);
-    m.total_work = d.work_end;
-    const uint32_t index = (uint32_t)m.hjobs.size();
-    pw_admit_kernel<<<1, 1, 0, m.stream>>>(m.st, m.ctx.jobs, m.ctx.job_done, d, index);
-    check(hipGetLastError(), "persistent admit");
     Impl::HostJob hj;
+    if (sbuf) {
+        d.sbuf = sbuf;
+        hj.owns_buffer = false;
+    } else {
+        check(hipMalloc(&d.sbuf, (size_t)work * sizeof(float4)), "hipMalloc sample buffer");
+        m.sbuf_bytes += (uint64_t)work * sizeof(float4);
+    }
+    m.total_work = d.work_end;
     hj.dev = d;
     hj.work = work;
     m.hjobs.push_back(hj);
+    return d;
+}
+
+int PersistentEngine::admit(const PersistentJob& job) {
+    Impl& m = *m_;
+    const uint32_t index = (uint32_t)m.hjobs.size();
+    const JobDev d = prepare_job(job, nullptr);
+    pw_admit_kernel<<<1, 1, 0, m.stream>>>(m.st, m.ctx.jobs, m.ctx.job_done, d, index);
+    check(hipGetLastError(), "persistent admit");
     return (int)index;
+}
+
+// Admits a chunk of jobs (an empty job table) with their sample buffers in the arena: one copy of
+// the job records and one kernel, instead of an allocation and a kernel per job.
+void PersistentEngine::admit_chunk(const std::vector<PersistentJob>& chunk) {
+    Impl& m = *m_;
+    uint64_t need = 0;
+    for (const PersistentJob& j : chunk) need += job_pixels(j) * (uint64_t)j.spp * sizeof(float4);
+    if (need > m.arena_bytes) {
+        m.free_arena();
+        check(hipMalloc(&m.arena, (size_t)need), "hipMalloc sample arena");
+        m.arena_bytes = need;
+    }
+    const uint32_t first = (uint32_t)m.hjobs.size();
+    m.staged.clear();
+    uint64_t off = 0;
+    for (const PersistentJob& j : chunk) {
+        m.staged.push_back(prepare_job(j, m.arena + off));
+        off += job_pixels(j) * (uint64_t)j.spp;
+    }
+    const uint32_t n = (uint32_t)m.staged.size();
+    check(hipMemcpyAsync(m.ctx.jobs + first, m.staged.data(), n * sizeof(JobDev), hipMemcpyHostToDevice, m.stream),
+          "copy job records");
+    pw_admit_many_kernel<<<1, 256, 0, m.stream>>>(m.st, m.ctx.job_done, first, n, m.total_work);
+    check(hipGetLastError(), "persistent admit");
 }
 
 PersistentEngine::LaunchPlan PersistentEngine::make_plan(bool single) {
@@ -1012,22 +1141,18 @@ bool PersistentEngine::job_complete(int job) const {
     return (uint64_t)j.done == j.work;
 }
 
-void PersistentEngine::finish_job(int job, bool keep_buffer) {
+void PersistentEngine::finish_job(int job) {
     Impl& m = *m_;
     Impl::HostJob& j = m.hjobs[job];
     if (j.finished) return;
     const int WH = j.dev.width * j.dev.height;
     pw_resolve_kernel<<<(WH + 255) / 256, 256, 0, m.stream>>>(j.dev);
     check(hipGetLastError(), "persistent resolve");
-    const uint64_t bytes = (uint64_t)WH * j.dev.spp * sizeof(float4);
-    if (keep_buffer && !m.spare) {
-        m.spare = j.dev.sbuf;   // stream order: the next job writes it after this resolve
-        m.spare_bytes = bytes;
-    } else {
+    if (j.owns_buffer) {
         check(hipStreamSynchronize(m.stream), "persistent resolve sync");
         check(hipFree(j.dev.sbuf), "hipFree sample buffer");
+        m.sbuf_bytes -= (uint64_t)WH * j.dev.spp * sizeof(float4);
     }
-    m.sbuf_bytes -= bytes;
     j.dev.sbuf = nullptr;
     j.finished = true;
 }
@@ -1065,7 +1190,7 @@ void PersistentEngine::render(const PersistentJob* jobs, int n_jobs, const Persi
         }
         if (chunk.empty()) break;
         if (res.chunks > 0) reset_jobs();
-        for (const PersistentJob& c : chunk) admit(c);
+        admit_chunk(chunk);
         PersistentProgress pr;
         if (stages) {
             run_profiled(stages);
@@ -1081,7 +1206,7 @@ void PersistentEngine::render(const PersistentJob* jobs, int n_jobs, const Persi
         res.iterations += (int)pr.iterations;
         for (int k = 0; k < (int)m_->hjobs.size(); ++k) {
             if (!job_complete(k)) throw std::runtime_error("persistent: job incomplete after the work drained");
-            finish_job(k, true);
+            finish_job(k);
         }
         res.chunks += 1;
     }

@@ -95,7 +95,14 @@ class BatchStats(C.Structure):
                 ("invalid_hits", C.c_int64), ("queue_overflows", C.c_int64), ("iterations", C.c_int64),
                 ("graph_launches", C.c_int64), ("block_size", C.c_int), ("pool_paths", C.c_int),
                 ("lds_kib", C.c_int), ("refill", C.c_int), ("leaf_batch", C.c_int),
-                ("dir_bins", C.c_int), ("shade_block", C.c_int), ("device_bytes", C.c_uint64)]
+                ("dir_bins", C.c_int), ("shade_block", C.c_int), ("device_bytes", C.c_uint64),
+                ("stage_ms", C.c_double * 8), ("stage_items", C.c_int64 * 8), ("stage_launches", C.c_int64 * 8),
+                ("stage_bytes", C.c_int64 * 8)]
+
+    def stages(self):
+        return {n: {"ms": self.stage_ms[i], "items": self.stage_items[i], "launches": self.stage_launches[i],
+                    "bytes": self.stage_bytes[i]}
+                for i, n in enumerate(RenderStats.STAGES) if self.stage_launches[i] > 0}
 
 
 class ServerStats(C.Structure):

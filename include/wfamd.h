@@ -234,6 +234,12 @@ typedef struct wf_batch_stats {
     int64_t iterations, graph_launches;
     int block_size, pool_paths, lds_kib, refill, leaf_batch, dir_bins, shade_block;
     uint64_t device_bytes;
+    /* With tuning->stage_timing: per-stage time, items, launches and algorithmic device bytes in
+     * wf_render_stats' order (extend, -, shade, -, -, shadow, -, other); otherwise zero. */
+    double stage_ms[8];
+    int64_t stage_items[8];
+    int64_t stage_launches[8];
+    int64_t stage_bytes[8];
 } wf_batch_stats;
 
 int wf_batch_render(const wf_job* jobs, int n_jobs, const wf_render_params* tuning, wf_batch_stats* stats);
