@@ -135,9 +135,11 @@ struct SceneCache {
 struct DevFilmAlloc {
     wf::gpu::DeviceFilm film{};
     int pixels = 0;
+    hipStream_t owner = nullptr;   // stream-ordered allocation (serving loop)
     void alloc(int px, hipStream_t stream) {
         pixels = px;
-        wf::gpu::check(hipMalloc(&film.rgb, (size_t)px * 5 * sizeof(float)), "hipMalloc film");
+        owner = stream;
+        wf::gpu::check(hipMallocAsync((void**)&film.rgb, (size_t)px * 5 * sizeof(float), stream), "hipMallocAsync film");
         film.lum = film.rgb + 3 * (size_t)px;
         film.lum2 = film.lum + px;
         film.pixels = px;
@@ -152,7 +154,7 @@ struct DevFilmAlloc {
                           lum_var);
     }
     void release() {
-        if (film.rgb) (void)hipFree(film.rgb);
+        if (film.rgb) (void)(owner ? hipFreeAsync(film.rgb, owner) : hipFree(film.rgb));
         film = wf::gpu::DeviceFilm{};
     }
 };
@@ -249,6 +251,7 @@ int wf_scene_orbit(wf_scene* scene, float yaw_deg) {
     return guard([&] {
         if (!scene) throw std::invalid_argument("null scene");
         wf::orbit_camera(scene->s, yaw_deg);
+        scene->info_valid = false;
         return (int)WF_OK;
     });
 }
@@ -272,6 +275,10 @@ void wf_scene_destroy(wf_scene* scene) { delete scene; }
 int wf_scene_info_get(const wf_scene* scene, wf_scene_info* out) {
     return guard([&] {
         if (!scene || !out) throw std::invalid_argument("null argument");
+        if (scene->info_valid) {
+            *out = scene->info;
+            return (int)WF_OK;
+        }
         const wf::Scene& s = scene->s;
         std::memset(out, 0, sizeof(*out));
         out->width = s.width;
@@ -288,6 +295,8 @@ int wf_scene_info_get(const wf_scene* scene, wf_scene_info* out) {
         out->bvh_max_depth = st.max_depth;
         out->bvh_sah_cost = st.sah_cost;
         out->hash = wf::scene_hash(s);
+        scene->info = *out;
+        scene->info_valid = true;
         return (int)WF_OK;
     });
 }

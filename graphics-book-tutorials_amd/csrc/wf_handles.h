@@ -2,10 +2,15 @@
 #pragma once
 #include <string>
 
+#include "../../include/wfamd.h"
 #include "wf_scene.h"
 
 struct wf_scene {
     wf::Scene s;
+    // wf_scene_info of s, computed on first request (the BVH statistics and the scene hash walk
+    // every node and primitive); cleared when the scene changes (wf_scene_orbit).
+    mutable bool info_valid = false;
+    mutable wf_scene_info info{};
 };
 
 namespace wf {

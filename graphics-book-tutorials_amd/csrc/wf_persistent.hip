@@ -731,7 +731,7 @@ struct PersistentEngine::Impl {
     ~Impl() {
         free_graph();
         for (HostJob& j : hjobs)
-            if (j.dev.sbuf && j.owns_buffer) (void)hipFree(j.dev.sbuf);
+            if (j.dev.sbuf && j.owns_buffer) (void)hipFree(j.dev.sbuf);   // the caller's stream may be gone
         free_arena();
         free_pool();
         if (st_host) (void)hipHostFree(st_host);
@@ -759,6 +759,12 @@ void PersistentEngine::ensure_pool(const PersistentTuning& t) {
         check(hipDeviceGetAttribute(&m.cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
         check(hipDeviceGetAttribute(&m.lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev),
               "hipDeviceGetAttribute");
+        // Sample buffers of admitted jobs come from the device's stream-ordered pool; keep freed
+        // memory in the pool for the next jobs instead of returning it at every synchronisation.
+        hipMemPool_t pool = nullptr;
+        check(hipDeviceGetDefaultMemPool(&pool, dev), "hipDeviceGetDefaultMemPool");
+        uint64_t keep = ~0ull;
+        check(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep), "hipMemPoolSetAttribute");
         for (const void* fn : {(const void*)pw_extend_kernel<true>, (const void*)pw_extend_kernel<false>,
                                (const void*)pw_shadow_kernel<true>, (const void*)pw_shadow_kernel<false>}) {
             hipFuncAttributes a{};
@@ -853,7 +859,7 @@ void PersistentEngine::open(const PersistentTuning& t, DevCounters* ctr, hipStre
 void PersistentEngine::reset_jobs() {
     Impl& m = *m_;
     for (Impl::HostJob& j : m.hjobs)
-        if (j.dev.sbuf && j.owns_buffer) (void)hipFree(j.dev.sbuf);
+        if (j.dev.sbuf && j.owns_buffer) (void)hipFreeAsync(j.dev.sbuf, m.stream);
     m.hjobs.clear();
     m.sbuf_bytes = 0;
     m.total_work = 0;
@@ -949,7 +955,9 @@ JobDev PersistentEngine::prepare_job(const PersistentJob& job, float4* sbuf) {
         d.sbuf = sbuf;
         hj.owns_buffer = false;
     } else {
-        check(hipMalloc(&d.sbuf, (size_t)work * sizeof(float4)), "hipMalloc sample buffer");
+        // Stream-ordered: no device-wide synchronisation, and finish_job's free returns the memory
+        // to the pool for the next job in stream order.
+        check(hipMallocAsync((void**)&d.sbuf, (size_t)work * sizeof(float4), m.stream), "hipMallocAsync sample buffer");
         m.sbuf_bytes += (uint64_t)work * sizeof(float4);
     }
     m.total_work = d.work_end;
@@ -1149,8 +1157,7 @@ void PersistentEngine::finish_job(int job) {
     pw_resolve_kernel<<<(WH + 255) / 256, 256, 0, m.stream>>>(j.dev);
     check(hipGetLastError(), "persistent resolve");
     if (j.owns_buffer) {
-        check(hipStreamSynchronize(m.stream), "persistent resolve sync");
-        check(hipFree(j.dev.sbuf), "hipFree sample buffer");
+        check(hipFreeAsync(j.dev.sbuf, m.stream), "hipFreeAsync sample buffer");   // after the resolve
         m.sbuf_bytes -= (uint64_t)WH * j.dev.spp * sizeof(float4);
     }
     j.dev.sbuf = nullptr;

@@ -79,6 +79,20 @@ def test_scene_view_equals_a_fresh_orbited_scene(wf):
         base.view(10, 9000)
 
 
+def test_scene_info_is_cached_and_follows_orbit(wf):
+    # The handle caches its info (BVH statistics and hash walk the whole scene); an orbit must
+    # change the cached hash exactly as a freshly orbited scene's.
+    s = wf.Scene("cornell555", 32, 18)
+    h0 = s.info().hash
+    assert s.info().hash == h0
+    s.orbit(40.0)
+    fresh = wf.Scene("cornell555", 32, 18)
+    fresh.orbit(40.0)
+    assert s.info().hash != h0
+    assert s.info().hash == fresh.info().hash
+    assert s.info().n_nodes == fresh.info().n_nodes and s.info().bvh_sah_cost == fresh.info().bvh_sah_cost
+
+
 # ---------------------------------------------------------------------------------------------
 # RNG and trigonometry
 # ---------------------------------------------------------------------------------------------
