@@ -4,6 +4,7 @@
 #pragma once
 #include <algorithm>
 #include <charconv>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -70,7 +71,7 @@ inline Config parse_args(int argc, const char* const* argv) {
         if (f == "--scene") c.scene = val();
         else if (f == "--width") c.width = num(1, 8192, "value outside 1..8192");
         else if (f == "--height") c.height = num(1, 8192, "value outside 1..8192");
-        else if (f == "--spp") c.spp = num(1, 1 << 30, "value < 1");
+        else if (f == "--spp") c.spp = num(1, INT_MAX, "value < 1");
         else if (f == "--seed") {
             if (!parse_uint(val(), c.seed)) throw ConfigError("--seed: non-numeric value");
         } else if (f == "--max-depth") c.max_depth = num(1, 64, "value outside 1..64");
@@ -84,8 +85,8 @@ inline Config parse_args(int argc, const char* const* argv) {
             int v;
             if (!parse_int(val(), v)) throw ConfigError("--threads: non-numeric value");
             c.threads = v;
-        } else if (f == "--reps") c.reps = num(1, 1 << 30, "value < 1");
-        else if (f == "--warmup") c.warmup = num(0, 1 << 30, "value < 0");
+        } else if (f == "--reps") c.reps = num(1, INT_MAX, "value < 1");
+        else if (f == "--warmup") c.warmup = num(0, INT_MAX, "value < 0");
         else throw ConfigError("unknown flag: " + f);
     }
     return c;
