@@ -807,6 +807,14 @@ int wf_server_create(uint64_t budget_bytes, const wf_render_params* tuning, wf_s
         if (!out) throw std::invalid_argument("null argument");
         auto s = std::make_unique<wf_server>();
         s->tune = to_tuning(tuning);
+        // Sample buffers and films of requests come from the device's stream-ordered pool; keep
+        // freed memory there for the next requests instead of returning it at every synchronisation.
+        int dev = 0;
+        wf::gpu::check(hipGetDevice(&dev), "hipGetDevice");
+        hipMemPool_t pool = nullptr;
+        wf::gpu::check(hipDeviceGetDefaultMemPool(&pool, dev), "hipDeviceGetDefaultMemPool");
+        uint64_t keep = ~0ull;
+        wf::gpu::check(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep), "hipMemPoolSetAttribute");
         wf::gpu::check(hipStreamCreate(&s->stream), "hipStreamCreate");
         wf::gpu::check(hipMalloc(&s->ctr, sizeof(wf::gpu::DevCounters)), "hipMalloc counters");
         wf::gpu::check(hipMemsetAsync(s->ctr, 0, sizeof(wf::gpu::DevCounters), s->stream), "hipMemset counters");

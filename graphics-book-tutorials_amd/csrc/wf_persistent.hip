@@ -24,6 +24,9 @@ constexpr int kDefaultRefill = 32;
 constexpr int kDefaultLeafBatch = 16;
 constexpr int kDefaultDirBins = 1;
 constexpr int kDefaultShadeBlock = 512;
+// Allowance for the extend and shadow kernels' static LDS (the plan, segment tables and, in the
+// multi-job kernels, the job and scene tables and finished counts: under 7.5 KiB at gfx950).
+constexpr int kLdsStatic = 8 * 1024;
 constexpr uint32_t kIdle = 0xFFFFFFFFu;
 constexpr uint32_t kFinalFlag = 0x80000000u;
 
@@ -223,17 +226,6 @@ __device__ __forceinline__ uint32_t seg_pos(const uint32_t* prefix, const uint32
 __global__ void pw_init_kernel(PState* st) {
     PState z{};
     *st = z;
-}
-
-// Jobs [first, first + n) have had their records copied in; total: the work end of the last.
-__global__ void pw_admit_many_kernel(PState* st, uint32_t* job_done, uint32_t first, uint32_t n,
-                                     unsigned long long total) {
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) job_done[first + i] = 0u;
-    if (threadIdx.x == 0) {
-        st->n_jobs = first + n;
-        st->total_work = total;
-        st->done = 0;
-    }
 }
 
 __global__ void pw_admit_kernel(PState* st, JobDev* jobs, uint32_t* job_done, JobDev job, uint32_t index) {
@@ -666,7 +658,7 @@ struct PersistentEngine::Impl {
     DevCounters* ctr = nullptr;
     int block = kDefaultBlock, graph_iters = kDefaultGraphIters, lds_bytes = kDefaultLdsKiB * 1024, cus = 0;
     int lds_max = 0;      // LDS a workgroup may use on this device (bytes)
-    int lds_static = 0;   // the traversal kernels' own LDS (bytes)
+    int lds_static = 0;   // allowance for the traversal kernels' own LDS (bytes)
     int refill = kDefaultRefill;
     int leaf_batch = kDefaultLeafBatch;
     int dir_bins = kDefaultDirBins;
@@ -711,7 +703,6 @@ struct PersistentEngine::Impl {
     // render(): one allocation holds the sample buffers of a chunk of jobs; kept across renders.
     float4* arena = nullptr;
     uint64_t arena_bytes = 0;
-    std::vector<JobDev> staged;   // the device records of the last chunk (source of its copy)
 
     void free_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -759,18 +750,7 @@ void PersistentEngine::ensure_pool(const PersistentTuning& t) {
         check(hipDeviceGetAttribute(&m.cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
         check(hipDeviceGetAttribute(&m.lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev),
               "hipDeviceGetAttribute");
-        // Sample buffers of admitted jobs come from the device's stream-ordered pool; keep freed
-        // memory in the pool for the next jobs instead of returning it at every synchronisation.
-        hipMemPool_t pool = nullptr;
-        check(hipDeviceGetDefaultMemPool(&pool, dev), "hipDeviceGetDefaultMemPool");
-        uint64_t keep = ~0ull;
-        check(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep), "hipMemPoolSetAttribute");
-        for (const void* fn : {(const void*)pw_extend_kernel<true>, (const void*)pw_extend_kernel<false>,
-                               (const void*)pw_shadow_kernel<true>, (const void*)pw_shadow_kernel<false>}) {
-            hipFuncAttributes a{};
-            check(hipFuncGetAttributes(&a, fn), "hipFuncGetAttributes");
-            m.lds_static = std::max(m.lds_static, (int)a.sharedSizeBytes);
-        }
+        m.lds_static = kLdsStatic;
     }
     int cap = t.pool_paths > 0 ? t.pool_paths : kDefaultPool;
     cap = (cap + 1023) & ~1023;
@@ -976,8 +956,8 @@ int PersistentEngine::admit(const PersistentJob& job) {
     return (int)index;
 }
 
-// Admits a chunk of jobs (an empty job table) with their sample buffers in the arena: one copy of
-// the job records and one kernel, instead of an allocation and a kernel per job.
+// Admits a chunk of jobs with their sample buffers in the arena (one allocation, kept across
+// renders, instead of an allocation per job).
 void PersistentEngine::admit_chunk(const std::vector<PersistentJob>& chunk) {
     Impl& m = *m_;
     uint64_t need = 0;
@@ -987,18 +967,14 @@ void PersistentEngine::admit_chunk(const std::vector<PersistentJob>& chunk) {
         check(hipMalloc(&m.arena, (size_t)need), "hipMalloc sample arena");
         m.arena_bytes = need;
     }
-    const uint32_t first = (uint32_t)m.hjobs.size();
-    m.staged.clear();
     uint64_t off = 0;
     for (const PersistentJob& j : chunk) {
-        m.staged.push_back(prepare_job(j, m.arena + off));
+        const uint32_t index = (uint32_t)m.hjobs.size();
+        const JobDev d = prepare_job(j, m.arena + off);
         off += job_pixels(j) * (uint64_t)j.spp;
+        pw_admit_kernel<<<1, 1, 0, m.stream>>>(m.st, m.ctx.jobs, m.ctx.job_done, d, index);
+        check(hipGetLastError(), "persistent admit");
     }
-    const uint32_t n = (uint32_t)m.staged.size();
-    check(hipMemcpyAsync(m.ctx.jobs + first, m.staged.data(), n * sizeof(JobDev), hipMemcpyHostToDevice, m.stream),
-          "copy job records");
-    pw_admit_many_kernel<<<1, 256, 0, m.stream>>>(m.st, m.ctx.job_done, first, n, m.total_work);
-    check(hipGetLastError(), "persistent admit");
 }
 
 PersistentEngine::LaunchPlan PersistentEngine::make_plan(bool single) {
