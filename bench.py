@@ -31,9 +31,28 @@ HBM_PEAK_GBS = 8000.0
 # copy kernel of known size (FETCH_SIZE x2.0, WRITE_SIZE x1.0); source and commands:
 # profiles/persistent_sphere_grid_1280x720x64_counters.md. Valid for that workload only.
 MEASURED_TRAFFIC = {"sphere_grid 1280x720x64": {"extend": 96.5e6, "shade": 400.7e6, "shadow": 56.8e6}}
+# Wave-level VALU instructions per render (SQ_INSTS_VALU of a counter pass over 2 renders, halved) on
+# this workload, same source; taken on a build that issued slightly more instructions than the
+# current one (it still divided twice per tested sphere), so the issue fraction below is an upper
+# bound.
+MEASURED_VALU_PER_RENDER = {"sphere_grid 1280x720x64": {"extend": 5.75e9, "shade": 2.18e9, "shadow": 1.165e9}}
+# VALU issue peak of the device: one wave64 FP32 instruction per 4 cycles on each 16-lane SIMD.
+VALU_SIMDS, VALU_CLOCK_GHZ = 1024, 2.4
 # VALU lane utilisation of the kernels (same source, counter pass after the signed slab test).
 MEASURED_LANE_UTIL = {"sphere_grid 1280x720x64": {"extend": 0.534, "shadow": 0.479, "shade": 0.515,
                                                    "megakernel": 0.193}}
+
+
+def valu_issue(instr_per_render, stage_ms):
+    """The kernel's VALU issue rate against the device's: wave-level VALU instructions per render
+    over the stage's time per render, against VALU_SIMDS SIMDs issuing one wave64 instruction per
+    4 cycles."""
+    if not instr_per_render or stage_ms <= 0:
+        return None
+    achieved = instr_per_render / (stage_ms / 1e3)
+    peak = VALU_SIMDS * VALU_CLOCK_GHZ * 1e9 / 4.0
+    return {"achieved": achieved, "peak": peak, "unit": "wave-instr/s", "frac": achieved / peak,
+            "note": "instruction count from a counter pass on an earlier build; an upper bound"}
 
 
 def percentile(v, p):
@@ -189,9 +208,10 @@ def main():
         traffic = MEASURED_TRAFFIC.get(wl, {}).get(dom)
         roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                "note": ("extend and shadow are BVH traversal: bound by LDS-latency node visits and lane "
-                         "divergence, not HBM; their VALU lane utilisation is given in lane_util"),
+                "note": ("extend and shadow are BVH traversal: bound by VALU issue of their node visits and "
+                         "leaf tests (valu_issue) at the lane utilisation given in lane_util, not by HBM"),
                 "lane_util": MEASURED_LANE_UTIL.get(wl),
+                "valu_issue": valu_issue(MEASURED_VALU_PER_RENDER.get(wl, {}).get(dom), d["ms"]),
                 "avg_launch_us": avg_s * 1e6, "bytes_per_launch": d["bytes"] / launches,
                 "items_per_launch": d["items"] / launches,
                 "stages": {k: {"ms": v["ms"], "launches": v["launches"], "items": v["items"],
