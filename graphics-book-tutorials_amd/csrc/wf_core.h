@@ -126,6 +126,29 @@ WF_HD void sincos_f32(float xf, float* s_out, float* c_out) {
     *c_out = (float)c;
 }
 
+// Unsigned 32-bit division by an invariant divisor d >= 1 as a multiply-high and shifts
+// (Granlund and Montgomery, "Division by invariant integers using multiplication", 1994: the
+// round-up multiplier with the add-back step): with l = ceil(log2 d) and
+// m = floor(2^32 (2^l - d) / d) + 1, n / d = (t + ((n - t) >> 1)) >> (l - 1) with t = mulhi(n, m),
+// for every 32-bit n; d = 1 is passed through.
+struct DivU32 {
+    uint32_t m, shift, d, pad;
+};
+
+inline DivU32 make_divu32(uint32_t d) {
+    if (d <= 1) return DivU32{0u, 0u, 1u, 0u};
+    uint32_t l = 0;
+    while ((1ull << l) < d) ++l;
+    const unsigned long long m = ((1ull << 32) * ((1ull << l) - d)) / d + 1;
+    return DivU32{(uint32_t)m, l - 1, d, 0u};
+}
+
+WF_HD uint32_t divu32(uint32_t n, const DivU32& v) {
+    const uint32_t t = (uint32_t)(((unsigned long long)n * v.m) >> 32);
+    const uint32_t q = (t + ((n - t) >> 1)) >> v.shift;
+    return v.d == 1u ? n : q;   // a select, so callers can load the divisor with their other data
+}
+
 // ---------------------------------------------------------------------------------------------
 // L0: counter-based RNG (include/wf/rng.hpp). Every draw is a pure function of
 // (seed, pixel, sample, depth, dim), so arms that schedule paths differently draw identical numbers.

@@ -78,6 +78,7 @@ struct JobDev {
     int width, height, max_depth;
     CameraRig cam;
     uint32_t first_sample, spp, seed, pad;
+    DivU32 div_pixels, div_band;   // by width * height and by 8 * width (new-path indexing)
     unsigned long long work_begin, work_end;
     float4* sbuf;
     DeviceFilm film;

@@ -114,34 +114,46 @@ __device__ __forceinline__ int job_depth(const PCtx& c, const JobTab* t, uint32_
 }
 
 // Pixel of position q in a pass, visiting the image in 8-row bands of 8x8 tiles (edge tiles
-// smaller), so the 64 consecutive new paths of a wave start from one 8x8 pixel tile.
-__device__ __forceinline__ void tile_pixel(uint32_t q, uint32_t W, uint32_t H, uint32_t* px, uint32_t* py) {
-    const uint32_t b = q / (8u * W);
+// smaller), so the 64 consecutive new paths of a wave start from one 8x8 pixel tile. Full tiles
+// (all but the last band and column) divide by shifts; band: division by 8 * W.
+__device__ __forceinline__ void tile_pixel(uint32_t q, uint32_t W, uint32_t H, const DivU32& band, uint32_t* px,
+                                           uint32_t* py) {
+    const uint32_t b = divu32(q, band);
     const uint32_t r = q - b * 8u * W;
     const uint32_t hb = min(8u, H - 8u * b);
-    const uint32_t j = r / (8u * hb);
+    const uint32_t j = hb == 8u ? r >> 6 : r / (8u * hb);
     const uint32_t in = r - j * 8u * hb;
     const uint32_t wb = min(8u, W - 8u * j);
-    *px = 8u * j + in % wb;
-    *py = 8u * b + in / wb;
+    *px = 8u * j + (wb == 8u ? in & 7u : in % wb);
+    *py = 8u * b + (wb == 8u ? in >> 3 : in / wb);
+}
+
+// Single-job kernels stage the job record in LDS once per workgroup: new paths read its camera,
+// size and divisors at every refill, which would otherwise be a round trip to memory each time.
+// The whole block calls it; the caller synchronises before use.
+__device__ __forceinline__ void stage_job0(const PCtx& c, JobDev* dst) {
+    const int* src = reinterpret_cast<const int*>(c.jobs);
+    int* d = reinterpret_cast<int*>(dst);
+    for (int i = threadIdx.x; i < (int)(sizeof(JobDev) / sizeof(int)); i += blockDim.x) d[i] = src[i];
 }
 
 // The path of work index w: its job, its sample-buffer index (sample-major, then pixel) and its
-// initial state (include/wf/path_core.hpp:59-82). t: the multi-job kernels' LDS tables.
+// initial state (include/wf/path_core.hpp:59-82). t: the multi-job kernels' LDS tables; job0: the
+// single-job kernels' LDS copy of job 0.
 template <bool SINGLE>
-__device__ __forceinline__ PathState new_path(const PCtx& c, const JobTab* t, uint32_t n_jobs, unsigned long long w,
-                                              uint32_t* jid_out, uint32_t* sidx_out) {
+__device__ __forceinline__ PathState new_path(const PCtx& c, const JobTab* t, const JobDev* job0, uint32_t n_jobs,
+                                              unsigned long long w, uint32_t* jid_out, uint32_t* sidx_out) {
     uint32_t jid = 0;
     if constexpr (!SINGLE) {
         if (t->staged) jid = (uint32_t)find_job([&](int k) { return t->work_end[k]; }, n_jobs, w);
         else jid = (uint32_t)find_job([&](int k) { return c.jobs[k].work_end; }, n_jobs, w);
     }
-    const JobDev& job = job_of<SINGLE>(c, jid);
+    const JobDev& job = SINGLE ? *job0 : c.jobs[jid];
     const uint32_t local = (uint32_t)(w - job.work_begin);
     const uint32_t W = (uint32_t)job.width, WH = W * (uint32_t)job.height;
-    const uint32_t q = local / WH;
+    const uint32_t q = divu32(local, job.div_pixels);
     uint32_t px, py;
-    tile_pixel(local - q * WH, W, (uint32_t)job.height, &px, &py);
+    tile_pixel(local - q * WH, W, (uint32_t)job.height, job.div_band, &px, &py);
     *jid_out = jid;
     *sidx_out = q * WH + py * W + px;
     return init_path_view(job.cam, job.width, job.height, (int)px, (int)py, job.first_sample + q, job.seed);
@@ -282,6 +294,8 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
     extern __shared__ Node lds[];
     __shared__ LdsPlan sp;
     __shared__ JobTab jt;   // multi-job only
+    __shared__ __align__(16) unsigned char job0_raw[sizeof(JobDev)];   // single-job only
+    JobDev& job0 = *reinterpret_cast<JobDev*>(job0_raw);
     const PState* st = c.st;
     const uint32_t n_act = st->it_act, n_live = st->it_live, n_jobs = st->n_jobs;
     const unsigned long long wbase = st->it_work_base;
@@ -290,6 +304,7 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
     if (threadIdx.x <= kSegs) pre[threadIdx.x] = st->in_prefix[threadIdx.x];
     if (threadIdx.x < kSegs) bas[threadIdx.x] = st->in_base[threadIdx.x];
     if constexpr (!SINGLE) stage_jobs(c, n_jobs, &jt);
+    else stage_job0(c, &job0);   // stage_lds synchronises
     stage_lds<SINGLE>(c, lds, &sp);   // synchronises
     WaveFeed f;
     feed_first(f, n_act);
@@ -322,7 +337,7 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
                     if constexpr (!SINGLE) sc = job_scene(c, &jt, in.kw[p].z);
                 } else {
                     uint32_t jid, sidx;
-                    r = new_path<SINGLE>(c, &jt, n_jobs, wbase + (item - n_live), &jid, &sidx).ray;
+                    r = new_path<SINGLE>(c, &jt, &job0, n_jobs, wbase + (item - n_live), &jid, &sidx).ray;
                     if constexpr (!SINGLE) sc = job_scene(c, &jt, jid);
                 }
                 if constexpr (!SINGLE) {
@@ -387,6 +402,8 @@ __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, Path
     __shared__ uint32_t pre[kSegs + 1], bas[kSegs], win[kSegs];
     __shared__ JobTab jt;                   // multi-job only
     __shared__ uint32_t done_cnt[kLdsJobs];   // multi-job only
+    __shared__ __align__(16) unsigned char job0_raw[sizeof(JobDev)];   // single-job only
+    JobDev& job0 = *reinterpret_cast<JobDev*>(job0_raw);
     PState* st = c.st;
     const uint32_t n = st->it_act, n_live = st->it_live, n_jobs = st->n_jobs;
     const unsigned long long wbase = st->it_work_base;
@@ -394,6 +411,7 @@ __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, Path
     const uint32_t stride = gridDim.x * blockDim.x;
     if (blockIdx.x * blockDim.x >= n) return;   // no round for this workgroup
     if constexpr (!SINGLE) stage_jobs(c, n_jobs, &jt, done_cnt);
+    else stage_job0(c, &job0);   // synchronised below
     if (threadIdx.x <= kSegs) pre[threadIdx.x] = st->in_prefix[threadIdx.x];
     if (threadIdx.x < kSegs) bas[threadIdx.x] = st->in_base[threadIdx.x];
     if (threadIdx.x == 0) seg_windows(n, blockDim.x, win);
@@ -432,10 +450,10 @@ __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, Path
                 jid = kw.z;
                 sidx = kw.w;
             } else {
-                ps = new_path<SINGLE>(c, &jt, n_jobs, wbase + (item - n_live), &jid, &sidx);
+                ps = new_path<SINGLE>(c, &jt, &job0, n_jobs, wbase + (item - n_live), &jid, &sidx);
             }
             key = ps.key;
-            const JobDev& job = job_of<SINGLE>(c, jid);
+            const JobDev& job = SINGLE ? job0 : c.jobs[jid];
             const SceneView* sv;
             int depth;
             if constexpr (SINGLE) {
@@ -929,6 +947,8 @@ JobDev PersistentEngine::prepare_job(const PersistentJob& job, float4* sbuf) {
     d.seed = job.seed;
     d.work_begin = m.total_work;
     d.work_end = m.total_work + work;
+    d.div_pixels = make_divu32((uint32_t)WH);
+    d.div_band = make_divu32(8u * (uint32_t)jw);
     d.film = job.film;
     Impl::HostJob hj;
     if (sbuf) {

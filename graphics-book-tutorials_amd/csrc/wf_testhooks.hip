@@ -76,6 +76,11 @@ __device__ __host__ inline void slab_case(const float* q, int* out) {
                  : -1;
 }
 
+__global__ void divu_kernel(const uint32_t* n, int count, DivU32 v, uint32_t* q) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) q[i] = divu32(n[i], v);
+}
+
 __global__ void slab_kernel(const float* q, int n, int* out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) slab_case(q + 14 * i, out + 2 * i);
@@ -225,6 +230,23 @@ int wf_test_sincos_grid_hash(uint32_t k0, uint32_t k1, int device, uint64_t* has
             h = fnv_words(h, (uint32_t)(p >> 32));
         }
         *hash = h;
+    });
+}
+
+int wf_test_divu32(const uint32_t* n, int count, uint32_t d, uint32_t* q, int device) {
+    return hook([&] {
+        if (d == 0 || count < 0 || (count > 0 && (!n || !q))) throw std::invalid_argument("bad argument");
+        const DivU32 v = make_divu32(d);
+        if (!device) {
+            for (int i = 0; i < count; ++i) q[i] = divu32(n[i], v);
+            return;
+        }
+        DevBuf<uint32_t> dn(count), dq(count);
+        dn.put(n, count);
+        if (count) divu_kernel<<<(count + 255) / 256, 256>>>(dn.p, count, v, dq.p);
+        check(hipGetLastError(), "divu kernel");
+        check(hipDeviceSynchronize(), "divu sync");
+        dq.get(q, count);
     });
 }
 

@@ -125,7 +125,7 @@ EXPORTS = [
     "wf_scan_workspace_destroy", "wf_scan_exclusive_u32", "wf_compact_u32", "wf_test_queue_push",
     "wf_batch_render", "wf_autotune", "wf_server_create", "wf_server_destroy", "wf_server_submit",
     "wf_server_step", "wf_server_poll", "wf_server_stats_get", "wf_test_sincos", "wf_test_sincos_grid_hash",
-    "wf_test_intersect", "wf_test_slab", "wf_test_trace_paths", "wf_copy_probe",
+    "wf_test_intersect", "wf_test_slab", "wf_test_divu32", "wf_test_trace_paths", "wf_copy_probe",
 ]
 
 
@@ -176,6 +176,7 @@ def lib():
         "wf_test_sincos_grid_hash": (I, [C.c_uint32, C.c_uint32, I, C.POINTER(C.c_uint64)]),
         "wf_test_intersect": (I, [P, F, I, I, F, C.POINTER(C.c_int), I]),
         "wf_test_slab": (I, [F, I, C.POINTER(C.c_int), I]),
+        "wf_test_divu32": (I, [C.POINTER(C.c_uint32), I, C.c_uint32, C.POINTER(C.c_uint32), I]),
         "wf_test_trace_paths": (I, [P, C.POINTER(C.c_int), I, C.c_uint32, I, F, C.POINTER(C.c_uint32), I]),
     }
     for name, (res, args) in sig.items():
@@ -418,6 +419,14 @@ def test_slab(cases, device=False):
     out = np.zeros((len(cases), 2), np.int32)
     _check(lib().wf_test_slab(_fptr(cases), len(cases), _iptr(out), int(device)))
     return out
+
+
+def test_divu32(n, d, device=False):
+    n = np.ascontiguousarray(n, np.uint32)
+    q = np.zeros(len(n), np.uint32)
+    _check(lib().wf_test_divu32(n.ctypes.data_as(C.POINTER(C.c_uint32)), len(n), int(d),
+                                q.ctypes.data_as(C.POINTER(C.c_uint32)), int(device)))
+    return q
 
 
 def test_intersect(scene, rays, anyhit=False, device=False):

@@ -200,6 +200,9 @@ int wf_test_intersect(const wf_scene* scene, const float* rays, int n, int anyhi
  * out[2i+1] = the per-ray-signed form the GPU traversal uses, or -1 where that form does not apply
  * (an infinite or zero 1/d component, a non-finite origin, a NaN tmin or tmax). */
 int wf_test_slab(const float* cases, int n, int* out, int device);
+/* Quotients n[i] / d by the multiply-high division by an invariant divisor that the persistent
+ * engine uses for new-path indexing (no reference counterpart; checked against plain division). */
+int wf_test_divu32(const uint32_t* n, int count, uint32_t d, uint32_t* q, int device);
 /* Radiance of single paths (px, py, sample triples) with the shared estimator, and per-path ray
  * counts (extension, shadow). */
 int wf_test_trace_paths(const wf_scene* scene, const int* pxys, int n, uint32_t seed, int max_depth, float* L,
