@@ -183,16 +183,21 @@ struct LdsNodes {
 
 // Wave-scheduled traversal. The per-ray sequence is exactly traversal_step's (same nodes, same
 // primitive tests, same order), split into two phases so that a wave runs its expensive leaf
-// tests together: visit() moves a lane through one node and, when the node is a hit leaf, parks
-// the leaf in `leaf` (code first << 2 | count - 1) with the continuation already in tr.node;
-// leaf() tests a parked leaf's primitives. A lane is live while tr.node >= 0 or leaf >= 0.
+// tests together: a visit (trav_visit, trav_visit_signed) moves a lane through one node and, when
+// the node is a hit leaf, parks the leaf in `leaf` as the node stores it, child = -(code + 1)
+// with code = first << 2 | count - 1 (so a parked leaf is negative, and kNoLeaf is none), with
+// the continuation already in tr.node; trav_leaf tests a parked leaf's primitives. A lane is live
+// while tr.node >= 0 or a leaf is parked.
+constexpr int kNoLeaf = 0;
+__device__ __forceinline__ bool leaf_parked(int leaf) { return leaf < 0; }
+
 template <class Src>
 __device__ __forceinline__ void trav_visit(int n_nodes, const Src& src, Traversal& tr, int& leaf) {
     const Node node = src.node(tr.node);
     int n;
     if (hit_box(node.lo, node.hi, tr.r.o, tr.inv_d, tr.r.tmin, tr.r.tmax)) {
         if (node.child < 0) {
-            leaf = -(node.child + 1);
+            leaf = node.child;
             n = node.skip;
         } else {
             n = node.child;
@@ -205,8 +210,9 @@ __device__ __forceinline__ void trav_visit(int n_nodes, const Src& src, Traversa
 
 template <class Src>
 __device__ __forceinline__ void trav_leaf(const Src& src, Traversal& tr, int& leaf, bool anyhit) {
-    const int first = leaf >> 2, cnt = (leaf & 3) + 1;
-    leaf = -1;
+    const int code = ~leaf;   // -(child + 1)
+    const int first = code >> 2, cnt = (code & 3) + 1;
+    leaf = kNoLeaf;
     for (int i = 0; i < cnt; ++i) {
         float t;
         if (src.hit(first + i, tr.r, &t)) {
@@ -230,8 +236,9 @@ __device__ __forceinline__ void trav_visit_signed(const Src& src, Traversal& tr,
     if constexpr (ALL_LDS) node = src.node_lds(tr.node);
     else node = src.node(tr.node);
     const bool hit = hit_box_signed(node.lo, node.hi, tr.r.o, tr.inv_d, tr.r.tmin, tr.r.tmax, neg_x, neg_y, neg_z);
-    if (hit && node.child < 0) leaf = ~node.child;   // -(child + 1)
-    tr.node = (hit && node.child >= 0) ? node.child : node.skip;
+    const bool inner = node.child >= 0;   // one compare; its negation is a mask operation
+    if (hit && !inner) leaf = node.child;
+    tr.node = (hit && inner) ? node.child : node.skip;
 }
 
 // MODE 0: trav_visit (any ray); 1: trav_visit_signed; 2: trav_visit_signed with every node in LDS.
@@ -239,7 +246,7 @@ template <int MODE, class Src>
 __device__ __forceinline__ void wave_traverse_loop(int n_nodes, const Src& src, Traversal& tr, int& leaf, bool anyhit,
                                                    int limit, int leaf_batch, bool neg_x, bool neg_y, bool neg_z) {
     for (;;) {
-        const bool parked = leaf >= 0;
+        const bool parked = leaf_parked(leaf);
         const bool moving = tr.node >= 0 && !parked;
         const int np = __popcll(__ballot(parked));
         const int nm = __popcll(__ballot(moving));
@@ -263,7 +270,7 @@ __device__ __forceinline__ void wave_traverse(int n_nodes, const Src& src, Trave
                                               int limit, int leaf_batch) {
     limit = __builtin_amdgcn_readfirstlane(limit);
     leaf_batch = __builtin_amdgcn_readfirstlane(leaf_batch);
-    const bool live = tr.node >= 0 || leaf >= 0;
+    const bool live = tr.node >= 0 || leaf_parked(leaf);
     const bool finite = slab_signed_ok(tr.r.o, tr.inv_d, tr.r.tmin, tr.r.tmax);
     const bool neg_x = tr.inv_d.x < 0.0f, neg_y = tr.inv_d.y < 0.0f, neg_z = tr.inv_d.z < 0.0f;
     if (__ballot(live && !finite) != 0)
@@ -447,7 +454,7 @@ __device__ __forceinline__ void trace_items(uint32_t n, uint32_t slice, int n_no
     Traversal tr;
     tr.node = -1;
     tr.prim = -1;
-    int leaf = -1;
+    int leaf = kNoLeaf;
     for (;;) {
         const uint64_t idle = __ballot(item == kIdleItem);
         if (idle != 0 && f.cursor >= f.end && !f.dry) feed_next(f, n);
@@ -465,7 +472,7 @@ __device__ __forceinline__ void trace_items(uint32_t n, uint32_t slice, int n_no
             continue;
         }
         wave_traverse(n_nodes, src, tr, leaf, anyhit, (f.dry && f.cursor >= f.end) ? 1 : refill, leaf_batch);
-        if (item != kIdleItem && tr.node < 0 && leaf < 0) {
+        if (item != kIdleItem && tr.node < 0 && !leaf_parked(leaf)) {
             done(item, tag, tr);
             item = kIdleItem;
         }

@@ -318,7 +318,7 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
     Traversal tr;
     tr.node = -1;
     tr.prim = -1;
-    int leaf = -1;
+    int leaf = kNoLeaf;
     for (;;) {
         const uint64_t idle = __ballot(item == kIdle);
         if (idle != 0 && f.cursor >= f.end && !f.dry) feed_next(f, n_act);
@@ -354,7 +354,7 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
             continue;   // slice used up: take the next one
         }
         wave_traverse(nn, src, tr, leaf, false, (f.dry && f.cursor >= f.end) ? 1 : c.refill, c.leaf_batch);
-        if (item != kIdle && tr.node < 0 && leaf < 0) {
+        if (item != kIdle && tr.node < 0 && !leaf_parked(leaf)) {
             Hit h{tr.r.tmax, tr.prim};
             const SceneView* hs;
             if constexpr (SINGLE) hs = &c.scene0;
@@ -584,7 +584,7 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
     Traversal tr;
     tr.node = -1;
     tr.prim = -1;
-    int leaf = -1;
+    int leaf = kNoLeaf;
     for (;;) {
         const uint64_t idle = __ballot(item == kIdle);
         if (idle != 0 && f.cursor >= f.end && !f.dry) feed_next(f, n);
@@ -617,7 +617,7 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
         wave_traverse(nn, src, tr, leaf, true, (f.dry && f.cursor >= f.end) ? 1 : c.refill, c.leaf_batch);
         bool fin = false;
         uint32_t jid = 0;
-        if (item != kIdle && tr.node < 0 && leaf < 0) {
+        if (item != kIdle && tr.node < 0 && !leaf_parked(leaf)) {
             fin = (tag & kFinalFlag) != 0;
             if constexpr (!SINGLE) jid = tag & ~kFinalFlag;
             if (tr.prim < 0) {
