@@ -166,12 +166,17 @@ struct PathKey {
     uint32_t k0, k1;
 };
 
+// k1 of a path's key from the seed's hash b = path_key_b(seed): a batched engine keeps b per job
+// and rebuilds k1 from the path's pixel and sample instead of storing it.
+WF_HD uint32_t path_key_b(uint32_t seed) { return hash_u32(seed + 0x85ebca6bU); }
+WF_HD uint32_t path_key_k1(uint32_t b, uint32_t pixel, uint32_t sample) {
+    return hash_u32(hash_u32(b ^ (pixel * 0x9e3779b1U)) + (sample * 0x85ebca77U + 0xc2b2ae3dU));
+}
+
 WF_HD PathKey make_path_key(uint32_t seed, uint32_t pixel, uint32_t sample) {
     uint32_t a = hash_u32(seed ^ 0x9e3779b9U);
-    uint32_t b = hash_u32(seed + 0x85ebca6bU);
     uint32_t k0 = hash_u32(hash_u32(a + pixel * 0xc2b2ae35U) ^ (sample * 0x27d4eb2fU + 0x165667b1U));
-    uint32_t k1 = hash_u32(hash_u32(b ^ (pixel * 0x9e3779b1U)) + (sample * 0x85ebca77U + 0xc2b2ae3dU));
-    return PathKey{k0, k1};
+    return PathKey{k0, path_key_k1(path_key_b(seed), pixel, sample)};
 }
 
 // Dimension slots per depth (< 16). 0-8 are the reference's (include/wf/rng.hpp:30-35);

@@ -77,7 +77,7 @@ struct JobDev {
     int scene;
     int width, height, max_depth;
     CameraRig cam;
-    uint32_t first_sample, spp, seed, pad;
+    uint32_t first_sample, spp, seed, key_b;   // key_b = path_key_b(seed)
     DivU32 div_pixels, div_band;   // by width * height and by 8 * width (new-path indexing)
     unsigned long long work_begin, work_end;
     float4* sbuf;
@@ -103,16 +103,17 @@ struct PState {
     uint32_t in_base[kSegs];
 };
 
-// One buffer of dense per-item path state (two of them alternate by iteration).
-//   ro : origin xyz, prev_pdf      rd : direction xyz, depth | specular << 16
-//   T  : throughput xyz            L  : radiance xyz
-//   kw : PathKey, job index, sample-buffer index
+// One buffer of dense per-item path state (two of them alternate by iteration), 64 bytes a path:
+//   ro : origin xyz, prev_pdf      rd : direction xyz, depth | specular << 16 | job << 17
+//   T  : throughput xyz, key k0    L  : radiance xyz, sample-buffer index
+// k1 of the path's key is rebuilt from the job's seed hash, the pixel and the sample, which the
+// sample-buffer index gives (path_key_k1). Depth <= 0xffff and job < kMaxJobs = 2^12 fit the word.
+static_assert(kMaxJobs <= (1 << 12), "the job index has 12 bits in a survivor's rd.w");
 struct PathBuf {
     float4* ro = nullptr;
     float4* rd = nullptr;
     float4* T = nullptr;
     float4* L = nullptr;
-    uint4* kw = nullptr;
 };
 
 // LDS layout of a workgroup: nodes of each scene, then primitive records of each scene.

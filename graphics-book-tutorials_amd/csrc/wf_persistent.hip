@@ -334,7 +334,7 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
                     r.d = Vec3{d.x, d.y, d.z};
                     r.tmin = kRayEps;
                     r.tmax = kInf;
-                    if constexpr (!SINGLE) sc = job_scene(c, &jt, in.kw[p].z);
+                    if constexpr (!SINGLE) sc = job_scene(c, &jt, (uint32_t)__float_as_int(d.w) >> 17);
                 } else {
                     uint32_t jid, sidx;
                     r = new_path<SINGLE>(c, &jt, &job0, n_jobs, wbase + (item - n_live), &jid, &sidx).ray;
@@ -374,7 +374,6 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
 struct ShadeIn {
     float2 hv;
     float4 o, d, t, l;
-    uint4 kw;
 };
 
 __device__ __forceinline__ void shade_load(const PCtx& c, const PathBuf& in, const uint32_t* pre, const uint32_t* bas,
@@ -387,7 +386,6 @@ __device__ __forceinline__ void shade_load(const PCtx& c, const PathBuf& in, con
         x->d = in.rd[p];
         x->t = in.T[p];
         x->l = in.L[p];
-        x->kw = in.kw[p];
     }
 }
 
@@ -435,20 +433,24 @@ __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, Path
             const float2 hv = cur.hv;
             if (item < n_live) {
                 const float4 o = cur.o, d = cur.d, t = cur.t, l = cur.l;
-                const uint4 kw = cur.kw;
                 ps.ray.o = Vec3{o.x, o.y, o.z};
                 ps.ray.d = Vec3{d.x, d.y, d.z};
                 ps.ray.tmin = kRayEps;
                 ps.ray.tmax = kInf;
                 ps.prev_pdf = o.w;
-                const int bits = __float_as_int(d.w);
-                ps.depth = bits & 0xffff;
-                ps.specular = bits >> 16;
+                const uint32_t bits = (uint32_t)__float_as_int(d.w);
+                ps.depth = (int)(bits & 0xffffu);
+                ps.specular = (int)((bits >> 16) & 1u);
                 ps.throughput = Vec3{t.x, t.y, t.z};
                 ps.L = Vec3{l.x, l.y, l.z};
-                ps.key = PathKey{kw.x, kw.y};
-                jid = kw.z;
-                sidx = kw.w;
+                jid = bits >> 17;
+                sidx = (uint32_t)__float_as_int(l.w);
+                // k1 from the job's seed hash, the pixel and the sample (the sample-buffer index
+                // is sample offset * width * height + pixel).
+                const JobDev& jb = SINGLE ? job0 : c.jobs[jid];
+                const uint32_t q = divu32(sidx, jb.div_pixels);
+                const uint32_t pix = sidx - q * (uint32_t)jb.width * (uint32_t)jb.height;
+                ps.key = PathKey{(uint32_t)__float_as_int(t.w), path_key_k1(jb.key_b, pix, jb.first_sample + q)};
             } else {
                 ps = new_path<SINGLE>(c, &jt, &job0, n_jobs, wbase + (item - n_live), &jid, &sidx);
             }
@@ -514,10 +516,9 @@ __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, Path
             o = bases[round][0] + cn[bin][wave] + mbcnt64(ml);
             out.ro[o] = make_float4(ps.ray.o.x, ps.ray.o.y, ps.ray.o.z, ps.prev_pdf);
             out.rd[o] = make_float4(ps.ray.d.x, ps.ray.d.y, ps.ray.d.z,
-                                    __int_as_float(ps.depth | (ps.specular << 16)));
-            out.T[o] = make_float4(ps.throughput.x, ps.throughput.y, ps.throughput.z, 0.0f);
-            out.L[o] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
-            out.kw[o] = make_uint4(key.k0, key.k1, jid, sidx);
+                                    __int_as_float((int)((uint32_t)ps.depth | ((uint32_t)ps.specular << 16) | (jid << 17))));
+            out.T[o] = make_float4(ps.throughput.x, ps.throughput.y, ps.throughput.z, __int_as_float((int)key.k0));
+            out.L[o] = make_float4(ps.L.x, ps.L.y, ps.L.z, __int_as_float((int)sidx));
         }
         if (has) {
             const uint32_t e = bases[round][1] + cn[8][wave] + mbcnt64(ms);
@@ -827,7 +828,6 @@ void PersistentEngine::ensure_pool(const PersistentTuning& t) {
         alloc(&b.rd, P * 16);
         alloc(&b.T, P * 16);
         alloc(&b.L, P * 16);
-        alloc(&b.kw, P * 16);
     }
     alloc(&c.hit, P * 8);
     alloc(&c.SQ.o, P * 16);
@@ -945,6 +945,7 @@ JobDev PersistentEngine::prepare_job(const PersistentJob& job, float4* sbuf) {
     d.first_sample = (uint32_t)job.first_sample;
     d.spp = (uint32_t)job.spp;
     d.seed = job.seed;
+    d.key_b = path_key_b(job.seed);
     d.work_begin = m.total_work;
     d.work_end = m.total_work + work;
     d.div_pixels = make_divu32((uint32_t)WH);
@@ -1101,12 +1102,12 @@ void PersistentEngine::run_profiled(StageTimes* out) {
         out->items[5] += s.n_shadow;
         // Algorithmic bytes (PathBuf and ShadowQueue layouts; BVH and primitives come from LDS):
         //   extend  survivor ray 32 (new paths are generated), hit 8
-        //   shade   hit 8; survivor state 80 in, 80 out; sample 16 per finished path; shadow ray 48
+        //   shade   hit 8; survivor state 64 in, 64 out; sample 16 per finished path; shadow ray 48
         //   shadow  ray 32 (origin, direction); the contribution and the radiance read-modify-write
         //           of unoccluded rays are not counted
         const long long live = s.it_live, act = s.it_act, next = s.n_next, sh = s.n_shadow;
         out->bytes[0] += 32 * live + 8 * act;
-        out->bytes[2] += 8 * act + 80 * live + 80 * next + 16 * (act - next) + 48 * sh;
+        out->bytes[2] += 8 * act + 64 * live + 64 * next + 16 * (act - next) + 48 * sh;
         out->bytes[5] += 32 * sh;
         out->items[7] += 1;
         out->launches[0] += 1;
