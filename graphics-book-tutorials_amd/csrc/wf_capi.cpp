@@ -777,6 +777,10 @@ struct wf_server {
     uint64_t budget = 0, in_use = 0;
     wf::gpu::PersistentTuning tune;
     hipStream_t stream = nullptr;
+    // Finished requests are resolved and downloaded here while the next launch runs on `stream`.
+    hipStream_t side = nullptr;
+    int pending = -1;                 // snapshot slot of the last launch, not yet collected
+    wf::gpu::DevCounters seen{};      // counters as of the last collected snapshot
     wf::gpu::DevCounters* ctr = nullptr;
     SceneCache scenes;
     wf::gpu::PersistentEngine eng;
@@ -788,8 +792,11 @@ struct wf_server {
     int jobs_in_table = 0;
 
     ~wf_server() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (side) (void)hipStreamSynchronize(side);
         for (auto& kv : reqs) kv.second->film.release();
         if (ctr) (void)hipFree(ctr);
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -816,6 +823,7 @@ int wf_server_create(uint64_t budget_bytes, const wf_render_params* tuning, wf_s
         uint64_t keep = ~0ull;
         wf::gpu::check(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep), "hipMemPoolSetAttribute");
         wf::gpu::check(hipStreamCreate(&s->stream), "hipStreamCreate");
+        wf::gpu::check(hipStreamCreate(&s->side), "hipStreamCreate");
         wf::gpu::check(hipMalloc(&s->ctr, sizeof(wf::gpu::DevCounters)), "hipMalloc counters");
         wf::gpu::check(hipMemsetAsync(s->ctr, 0, sizeof(wf::gpu::DevCounters), s->stream), "hipMemset counters");
         s->eng.open(s->tune, s->ctr, s->stream);
@@ -856,8 +864,8 @@ int wf_server_step(wf_server* s, int launches) {
     return guard([&] {
         if (!s) throw std::invalid_argument("null server");
         auto t0 = std::chrono::steady_clock::now();
-        // When nothing runs, recycle the job table.
-        if (s->running.empty() && s->jobs_in_table > 0) {
+        // When nothing runs and no launch is outstanding, recycle the job table.
+        if (s->running.empty() && s->pending < 0 && s->jobs_in_table > 0) {
             s->eng.reset_jobs();
             s->jobs_in_table = 0;
         }
@@ -885,14 +893,27 @@ int wf_server_step(wf_server* s, int launches) {
             s->running.push_back(r);
             s->admitted++;
         }
+        // Queue this step's launches first, then finish what the previous step's launches
+        // completed: the host work below overlaps the launches just queued.
+        int launched = -1;
         if (!s->running.empty() && launches > 0) {
             s->eng.run(launches);
-            s->eng.progress();
+            launched = s->eng.snapshot(s->ctr);
+        }
+        if (s->pending >= 0) {
+            const int slot = s->pending;
+            s->eng.collect(slot, &s->seen);
+            bool waited = false;
             std::vector<wf_server::Req*> still;
             for (wf_server::Req* r : s->running) {
                 if (s->eng.job_complete(r->engine_job)) {
-                    s->eng.finish_job(r->engine_job);
-                    r->film.download(r->job.spp, r->job.rgb, r->job.lum_var, s->stream);
+                    if (!waited) {   // the side stream starts after the snapshot's launches
+                        wf::gpu::check(hipStreamWaitEvent(s->side, s->eng.snapshot_event(slot), 0), "hipStreamWaitEvent");
+                        waited = true;
+                    }
+                    s->eng.finish_job(r->engine_job, s->side);
+                    r->film.download(r->job.spp, r->job.rgb, r->job.lum_var, s->side);
+                    r->film.owner = s->side;
                     r->film.release();
                     r->state = 2;
                     s->in_use -= r->bytes;
@@ -903,6 +924,7 @@ int wf_server_step(wf_server* s, int launches) {
             }
             s->running.swap(still);
         }
+        s->pending = launched;
         s->busy_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         return (int)WF_OK;
     });
@@ -924,8 +946,13 @@ int wf_server_stats_get(wf_server* s, wf_server_stats* out) {
         out->completed = s->completed;
         out->rejected = s->rejected;
         out->in_flight = (int64_t)s->running.size();
-        wf::gpu::DevCounters c{};
-        wf::gpu::check(hipMemcpy(&c, s->ctr, sizeof(c), hipMemcpyDeviceToHost), "copy counters");
+        // While requests run, the counters of the last collected snapshot (reading the device
+        // counters would wait for the queued launch); when idle, the device counters.
+        wf::gpu::DevCounters c = s->seen;
+        if (s->running.empty()) {
+            wf::gpu::check(hipMemcpyAsync(&c, s->ctr, sizeof(c), hipMemcpyDeviceToHost, s->stream), "copy counters");
+            wf::gpu::check(hipStreamSynchronize(s->stream), "counters sync");
+        }
         out->extension_rays = c.extension_rays;
         out->shadow_rays = c.shadow_rays;
         out->bytes_in_use = s->in_use;

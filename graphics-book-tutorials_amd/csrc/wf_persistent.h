@@ -150,14 +150,23 @@ public:
     // sample buffer is allocated here) and returns its index; run() replays the captured
     // iteration graph `launches` times; progress() reads the control block and the per-job
     // finished-path counts; finish_job() resolves a complete job into its film and frees its
-    // sample buffer; reset_jobs() clears the job table once nothing is in flight.
+    // sample buffer (on `stream` when given, else the engine's); reset_jobs() clears the job
+    // table once nothing is in flight.
+    // For a caller that keeps a launch queued while it works on the host: snapshot() enqueues,
+    // after everything enqueued so far, copies of the control block, the per-job finished counts
+    // and *ctr (optional) into one of two pinned slots, records an event and returns the slot;
+    // collect() waits for that event only (not for later launches) and applies the finished
+    // counts of the jobs that existed at the snapshot. At most two snapshots are outstanding.
     void open(const PersistentTuning& t, DevCounters* ctr, hipStream_t stream);
     int admit(const PersistentJob& job);
     void run(int launches);
     void run_profiled(StageTimes* out);
     PersistentProgress progress();
+    int snapshot(const DevCounters* ctr);
+    PersistentProgress collect(int slot, DevCounters* ctr_out);
+    hipEvent_t snapshot_event(int slot) const;
     bool job_complete(int job) const;
-    void finish_job(int job);
+    void finish_job(int job, hipStream_t stream = nullptr);
     void reset_jobs();
     uint64_t pool_bytes() const { return pool_bytes_; }
     uint64_t bytes() const;

@@ -722,6 +722,18 @@ struct PersistentEngine::Impl {
     // render(): one allocation holds the sample buffers of a chunk of jobs; kept across renders.
     float4* arena = nullptr;
     uint64_t arena_bytes = 0;
+    // Progress snapshots (snapshot/collect): pinned copies and the event that marks them done.
+    struct Snap {
+        PState* st = nullptr;
+        uint32_t* done = nullptr;
+        DevCounters* ctr = nullptr;
+        hipEvent_t ev = nullptr;
+        size_t n_jobs = 0;
+        uint64_t jobs_gen = 0;
+        bool with_ctr = false, outstanding = false;
+    } snap[2];
+    int snap_next = 0;
+    uint64_t jobs_gen = 0;   // bumped by reset_jobs: a snapshot of an older job table applies to none
 
     void free_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -746,6 +758,13 @@ struct PersistentEngine::Impl {
         free_pool();
         if (st_host) (void)hipHostFree(st_host);
         if (done_host) (void)hipHostFree(done_host);
+        for (Snap& s : snap) {
+            if (s.ev) (void)hipEventSynchronize(s.ev);
+            if (s.ev) (void)hipEventDestroy(s.ev);
+            if (s.st) (void)hipHostFree(s.st);
+            if (s.done) (void)hipHostFree(s.done);
+            if (s.ctr) (void)hipHostFree(s.ctr);
+        }
     }
 };
 
@@ -859,6 +878,7 @@ void PersistentEngine::reset_jobs() {
     for (Impl::HostJob& j : m.hjobs)
         if (j.dev.sbuf && j.owns_buffer) (void)hipFreeAsync(j.dev.sbuf, m.stream);
     m.hjobs.clear();
+    m.jobs_gen += 1;
     m.sbuf_bytes = 0;
     m.total_work = 0;
     pw_init_kernel<<<1, 1, 0, m.stream>>>(m.st);
@@ -1141,20 +1161,67 @@ PersistentProgress PersistentEngine::progress() {
     return p;
 }
 
+int PersistentEngine::snapshot(const DevCounters* ctr) {
+    Impl& m = *m_;
+    if (!m.st) throw std::logic_error("persistent snapshot: engine not open");
+    const int slot = m.snap_next;
+    Impl::Snap& s = m.snap[slot];
+    if (s.outstanding) throw std::logic_error("persistent snapshot: both slots are waiting to be collected");
+    if (!s.st) {
+        check(hipHostMalloc(&s.st, sizeof(PState), hipHostMallocDefault), "hipHostMalloc");
+        check(hipHostMalloc(&s.done, sizeof(uint32_t) * kMaxJobs, hipHostMallocDefault), "hipHostMalloc");
+        check(hipHostMalloc(&s.ctr, sizeof(DevCounters), hipHostMallocDefault), "hipHostMalloc");
+        check(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming), "hipEventCreate");
+    }
+    s.n_jobs = m.hjobs.size();
+    s.jobs_gen = m.jobs_gen;
+    s.with_ctr = ctr != nullptr;
+    check(hipMemcpyAsync(s.st, m.st, sizeof(PState), hipMemcpyDeviceToHost, m.stream), "copy snapshot");
+    if (s.n_jobs)
+        check(hipMemcpyAsync(s.done, m.ctx.job_done, s.n_jobs * sizeof(uint32_t), hipMemcpyDeviceToHost, m.stream),
+              "copy snapshot");
+    if (ctr) check(hipMemcpyAsync(s.ctr, ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, m.stream), "copy snapshot");
+    check(hipEventRecord(s.ev, m.stream), "hipEventRecord snapshot");
+    s.outstanding = true;
+    m.snap_next ^= 1;
+    return slot;
+}
+
+PersistentProgress PersistentEngine::collect(int slot, DevCounters* ctr_out) {
+    Impl& m = *m_;
+    if (slot < 0 || slot > 1 || !m.snap[slot].outstanding) throw std::logic_error("persistent collect: no such snapshot");
+    Impl::Snap& s = m.snap[slot];
+    check(hipEventSynchronize(s.ev), "snapshot wait");
+    s.outstanding = false;
+    if (s.jobs_gen == m.jobs_gen)
+        for (size_t k = 0; k < s.n_jobs && k < m.hjobs.size(); ++k) m.hjobs[k].done = s.done[k];
+    if (ctr_out && s.with_ctr) *ctr_out = *s.ctr;
+    PersistentProgress p;
+    p.next_work = s.st->next_work;
+    p.total_work = s.st->total_work;
+    p.n_live = s.st->n_live;
+    p.done = s.st->done;
+    p.iterations = s.st->iterations;
+    return p;
+}
+
+hipEvent_t PersistentEngine::snapshot_event(int slot) const { return m_->snap[slot & 1].ev; }
+
 bool PersistentEngine::job_complete(int job) const {
     const Impl::HostJob& j = m_->hjobs[job];
     return (uint64_t)j.done == j.work;
 }
 
-void PersistentEngine::finish_job(int job) {
+void PersistentEngine::finish_job(int job, hipStream_t stream) {
     Impl& m = *m_;
     Impl::HostJob& j = m.hjobs[job];
     if (j.finished) return;
+    hipStream_t on = stream ? stream : m.stream;
     const int WH = j.dev.width * j.dev.height;
-    pw_resolve_kernel<<<(WH + 255) / 256, 256, 0, m.stream>>>(j.dev);
+    pw_resolve_kernel<<<(WH + 255) / 256, 256, 0, on>>>(j.dev);
     check(hipGetLastError(), "persistent resolve");
     if (j.owns_buffer) {
-        check(hipFreeAsync(j.dev.sbuf, m.stream), "hipFreeAsync sample buffer");   // after the resolve
+        check(hipFreeAsync(j.dev.sbuf, on), "hipFreeAsync sample buffer");   // after the resolve
         m.sbuf_bytes -= (uint64_t)WH * j.dev.spp * sizeof(float4);
     }
     j.dev.sbuf = nullptr;

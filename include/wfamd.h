@@ -263,11 +263,14 @@ int wf_server_create(uint64_t budget_bytes, const wf_render_params* tuning, wf_s
 void wf_server_destroy(wf_server* s);
 /* Queues a request; returns its id in *id. WF_ERR_BUDGET if it can never fit the budget. */
 int wf_server_submit(wf_server* s, const wf_job* job, int64_t* id);
-/* Admits queued requests that fit, advances the device work by `launches` graph replays, and
- * finishes completed requests (downloads their results). */
+/* Admits queued requests that fit, queues `launches` graph replays, and then finishes the requests
+ * that the previous step's replays completed (resolves and downloads their results on a second
+ * stream while the replays just queued run). A request is therefore reported done by the step
+ * after the one whose replays completed it; step(s, 0) only collects. */
 int wf_server_step(wf_server* s, int launches);
 /* 1 if request `id` is done (its buffers are filled), 0 if not yet, negative if unknown. */
 int wf_server_poll(wf_server* s, int64_t id);
+/* Ray counts are those of the last collected step while requests run, and exact when none do. */
 int wf_server_stats_get(wf_server* s, wf_server_stats* out);
 
 #ifdef __cplusplus
