@@ -807,6 +807,7 @@ void PersistentEngine::ensure_pool(const PersistentTuning& t) {
     m.ctx.refill = m.refill;
     m.ctx.leaf_batch = m.leaf_batch;
     m.ctx.dir_bins = m.dir_bins;
+    if (t.graph_iters > 1024) throw std::runtime_error("persistent: graph iterations must be 1..1024");
     m.graph_iters = t.graph_iters > 0 ? ((t.graph_iters + 1) & ~1) : kDefaultGraphIters;
     const int kib = t.lds_kib > 0 ? t.lds_kib : kDefaultLdsKiB;
     // The budget is the dynamic LDS; the kernels' own (the plan and the segment tables)

@@ -149,6 +149,7 @@ void GpuRenderer::run_passes(int first_sample, int count, const RenderParams& p,
             tune.leaf_batch = p.leaf_batch;
             tune.dir_bins = p.dir_bins;
             tune.shade_block = p.shade_block;
+            tune.graph_iters = p.graph_iters;
             gpu::PersistentResult pr;
             m.pe->render(&job, 1, tune, m.d_ctr, m.stream, &pr, p.stage_timing ? &st : nullptr);
             iterations = pr.iterations;

@@ -56,6 +56,8 @@ typedef struct wf_render_params {
     int leaf_batch;       /* persistent: parked leaves that start a wave's leaf-test round */
     int dir_bins;         /* persistent: survivor grouping, 1 none or 8 direction octants */
     int shade_block;      /* persistent: threads per shade workgroup = compaction round (128..1024) */
+    int graph_iters;      /* persistent: pool iterations per captured graph launch (0 = 16, or 8 in the
+                           * serving loop; rounded up to even) */
 } wf_render_params;
 
 /* RenderStats, include/wf/renderer.hpp:23-37. stage_ms: extend, classify, shade_lambert,

@@ -388,6 +388,39 @@ def test_serving_job_views_equal_standalone(wf, scenes):
     srv.close()
 
 
+def test_serving_counts_rays_exactly_and_closes_with_requests_in_flight(wf, scenes):
+    # A step collects the previous step's launches; once nothing runs, the counters are the
+    # device's and equal the standalone renders' ray counts. A server closed mid-flight waits for
+    # its streams before freeing.
+    s = scenes("sphere_grid", 64, 36)
+    srv = wf.Server(budget_bytes=8 << 30, tuning=wf.params(pool_paths=4096))
+    reqs = [(3, 21), (2, 22), (4, 23)]
+    ids = [srv.submit(s, spp, seed) for spp, seed in reqs]
+    for _ in range(2000):
+        srv.step(1)
+        if all(srv.poll(i) == 1 for i in ids):
+            break
+    assert all(srv.poll(i) == 1 for i in ids)
+    srv.step(0)   # collects the last launch; nothing left to finish
+    st = srv.stats()
+    assert st["in_flight"] == 0 and st["completed"] == len(reqs)
+    want = 0
+    for spp, seed in reqs:
+        r = wf.Renderer(s, "megakernel")
+        _, _, mst = r.render(wf.params(spp=spp, seed=seed))
+        r.close()
+        want += mst.extension_rays + mst.shadow_rays
+    assert st["extension_rays"] + st["shadow_rays"] == want
+    srv.close()
+    # One step admits and launches; a completion could only be seen by the next step.
+    srv = wf.Server(budget_bytes=8 << 30, tuning=wf.params(pool_paths=4096))
+    srv.submit(scenes("sphere_grid", 160, 90), 16, 30)
+    srv.submit(s, 8, 31)
+    srv.step(1)
+    assert srv.stats()["in_flight"] == 2
+    srv.close()
+
+
 def test_serving_budget_rejects_and_defers(wf, scenes):
     s = scenes("sphere_grid", 64, 36)
     probe = wf.Server(budget_bytes=1 << 40, tuning=wf.params(pool_paths=4096))
