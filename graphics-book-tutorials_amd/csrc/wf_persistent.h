@@ -189,6 +189,10 @@ private:
     std::unique_ptr<Impl> m_;
 };
 
+// LDS layout for scenes of n_nodes[k] nodes and n_prims[k] primitives within `budget` bytes (the
+// plan of a batch's workgroups).
+LdsPlan plan_lds(const int* n_nodes, const int* n_prims, int n, long long budget);
+
 // Picks block size, pool size and LDS budget for a workload by timing short renders of it.
 PersistentTuning autotune_persistent(const PersistentJob* jobs, int n_jobs, hipStream_t stream, int probe_spp,
                                      std::vector<std::pair<PersistentTuning, double>>* trials);

@@ -887,42 +887,65 @@ void PersistentEngine::reset_jobs() {
     check(hipGetLastError(), "persistent init");
 }
 
-// Splits the LDS budget over the scenes, smallest need first: each scene gets what it needs (all
-// of its nodes, then its primitive records) up to an even share of what is left, so small scenes
-// are served whole from LDS and large ones share the rest (their breadth-first tops).
+// Splits the LDS budget over the scenes. Smallest need first, a scene gets its whole tree (then
+// its primitive records, if they fit too) when that leaves kMinTopBytes for each other scene not
+// yet whole (those after it and those already left partial):
+// a whole tree runs the all-LDS visit, worth more than a few more top nodes of a large tree. The
+// scenes left over share the rest evenly and store the breadth-first top of their trees.
+constexpr long long kMinTopBytes = 4 * 1024;
+
+LdsPlan plan_lds(const int* n_nodes, const int* n_prims, int n, long long budget) {
+    if (n < 0 || n > kMaxScenes) throw std::invalid_argument("lds plan: scene count outside 0..8");
+    LdsPlan p{};
+    p.n = n;
+    if (n == 0) return p;
+    auto need = [&](int k) { return (long long)n_nodes[k] * (long long)sizeof(Node) + (long long)n_prims[k] * 16; };
+    std::vector<int> order(n);
+    for (int k = 0; k < n; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return need(a) < need(b); });
+    long long left = budget;
+    std::vector<int> rest;
+    for (int i = 0; i < n; ++i) {
+        const int k = order[i];
+        // What stays for the scenes after this one and for those already left partial.
+        const long long avail = left - kMinTopBytes * (long long)(n - 1 - i + (int)rest.size());
+        const long long nodes = (long long)n_nodes[k] * (long long)sizeof(Node);
+        if (nodes > avail) {
+            rest.push_back(k);
+            continue;
+        }
+        p.node_cnt[k] = n_nodes[k];
+        p.prim_cnt[k] = (long long)n_prims[k] * 16 <= avail - nodes ? n_prims[k] : 0;
+        left -= nodes + (long long)p.prim_cnt[k] * 16;
+    }
+    for (size_t i = 0; i < rest.size(); ++i) {
+        const int k = rest[i];
+        const long long share = left / (long long)(rest.size() - i);
+        p.node_cnt[k] = (int)std::min<long long>(n_nodes[k], std::max<long long>(share, 0) / (long long)sizeof(Node));
+        p.prim_cnt[k] = 0;
+        left -= (long long)p.node_cnt[k] * (long long)sizeof(Node);
+    }
+    int nodes = 0, prims = 0;
+    for (int k = 0; k < n; ++k) {
+        p.node_off[k] = nodes;
+        p.prim_off[k] = prims;
+        nodes += p.node_cnt[k];
+        prims += p.prim_cnt[k];
+    }
+    p.total_nodes = nodes;
+    p.total_prims = prims;
+    return p;
+}
+
 void PersistentEngine::update_lds_plan() {
     Impl& m = *m_;
-    LdsPlan p{};
-    p.n = (int)m.hscenes.size();
-    if (p.n > 0) {
-        auto need = [&](int k) {
-            const SceneView& s = m.hscenes[k];
-            return (long long)s.n_nodes * (long long)sizeof(Node) + (long long)s.n_prims * 16;
-        };
-        std::vector<int> order(p.n);
-        for (int k = 0; k < p.n; ++k) order[k] = k;
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return need(a) < need(b); });
-        long long left = m.lds_bytes;
-        for (int i = 0; i < p.n; ++i) {
-            const int k = order[i];
-            const SceneView& s = m.hscenes[k];
-            const long long share = left / (p.n - i);
-            p.node_cnt[k] = (int)std::min<long long>(s.n_nodes, share / (long long)sizeof(Node));
-            const long long rest = share - (long long)p.node_cnt[k] * (long long)sizeof(Node);
-            p.prim_cnt[k] = (p.node_cnt[k] == s.n_nodes && (long long)s.n_prims * 16 <= rest) ? s.n_prims : 0;
-            left -= (long long)p.node_cnt[k] * (long long)sizeof(Node) + (long long)p.prim_cnt[k] * 16;
-        }
-        int nodes = 0, prims = 0;
-        for (int k = 0; k < p.n; ++k) {
-            p.node_off[k] = nodes;
-            p.prim_off[k] = prims;
-            nodes += p.node_cnt[k];
-            prims += p.prim_cnt[k];
-        }
-        p.total_nodes = nodes;
-        p.total_prims = prims;
+    int nn[kMaxScenes] = {}, np[kMaxScenes] = {};
+    const int n = (int)m.hscenes.size();
+    for (int k = 0; k < n; ++k) {
+        nn[k] = m.hscenes[k].n_nodes;
+        np[k] = m.hscenes[k].n_prims;
     }
-    m.lds_plan = p;
+    m.lds_plan = plan_lds(nn, np, n, m.lds_bytes);
     check(hipMemcpyAsync(m.ctx.plan, &m.lds_plan, sizeof(LdsPlan), hipMemcpyHostToDevice, m.stream), "copy lds plan");
 }
 
@@ -1361,7 +1384,7 @@ PersistentTuning autotune_persistent(const PersistentJob* jobs, int n_jobs, hipS
     base = best;
     for (int p : pools) {
         if (p > top) continue;
-        for (int kib : {32, 64, 128}) {
+        for (int kib : {32, 40, 48, 64, 128}) {
             if (p == base.pool_paths && kib == base.lds_kib) continue;
             PersistentTuning t = base;
             t.pool_paths = p;

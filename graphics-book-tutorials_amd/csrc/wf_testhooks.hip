@@ -8,6 +8,7 @@
 #include "../../include/wfamd.h"
 #include "wf_gpu.h"
 #include "wf_handles.h"
+#include "wf_persistent.h"
 
 namespace wf::gpu {
 namespace {
@@ -230,6 +231,18 @@ int wf_test_sincos_grid_hash(uint32_t k0, uint32_t k1, int device, uint64_t* has
             h = fnv_words(h, (uint32_t)(p >> 32));
         }
         *hash = h;
+    });
+}
+
+int wf_test_lds_plan(const int* n_nodes, const int* n_prims, int n, int64_t budget_bytes, int* node_cnt,
+                     int* prim_cnt) {
+    return hook([&] {
+        if (n < 0 || (n > 0 && (!n_nodes || !n_prims || !node_cnt || !prim_cnt))) throw std::invalid_argument("bad argument");
+        const LdsPlan p = plan_lds(n_nodes, n_prims, n, budget_bytes);
+        for (int k = 0; k < n; ++k) {
+            node_cnt[k] = p.node_cnt[k];
+            prim_cnt[k] = p.prim_cnt[k];
+        }
     });
 }
 

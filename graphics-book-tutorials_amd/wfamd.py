@@ -125,7 +125,7 @@ EXPORTS = [
     "wf_scan_workspace_destroy", "wf_scan_exclusive_u32", "wf_compact_u32", "wf_test_queue_push",
     "wf_batch_render", "wf_autotune", "wf_server_create", "wf_server_destroy", "wf_server_submit",
     "wf_server_step", "wf_server_poll", "wf_server_stats_get", "wf_test_sincos", "wf_test_sincos_grid_hash",
-    "wf_test_intersect", "wf_test_slab", "wf_test_divu32", "wf_test_trace_paths", "wf_copy_probe",
+    "wf_test_intersect", "wf_test_slab", "wf_test_divu32", "wf_test_lds_plan", "wf_test_trace_paths", "wf_copy_probe",
 ]
 
 
@@ -177,6 +177,8 @@ def lib():
         "wf_test_intersect": (I, [P, F, I, I, F, C.POINTER(C.c_int), I]),
         "wf_test_slab": (I, [F, I, C.POINTER(C.c_int), I]),
         "wf_test_divu32": (I, [C.POINTER(C.c_uint32), I, C.c_uint32, C.POINTER(C.c_uint32), I]),
+        "wf_test_lds_plan": (I, [C.POINTER(C.c_int), C.POINTER(C.c_int), I, C.c_int64, C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int)]),
         "wf_test_trace_paths": (I, [P, C.POINTER(C.c_int), I, C.c_uint32, I, F, C.POINTER(C.c_uint32), I]),
     }
     for name, (res, args) in sig.items():
@@ -427,6 +429,18 @@ def test_divu32(n, d, device=False):
     _check(lib().wf_test_divu32(n.ctypes.data_as(C.POINTER(C.c_uint32)), len(n), int(d),
                                 q.ctypes.data_as(C.POINTER(C.c_uint32)), int(device)))
     return q
+
+
+def test_lds_plan(n_nodes, n_prims, budget_bytes):
+    """(nodes, primitive records) each scene keeps in a persistent workgroup's LDS."""
+    nn = np.ascontiguousarray(n_nodes, np.int32)
+    np_ = np.ascontiguousarray(n_prims, np.int32)
+    nc = np.zeros(len(nn), np.int32)
+    pc = np.zeros(len(nn), np.int32)
+    _check(lib().wf_test_lds_plan(nn.ctypes.data_as(C.POINTER(C.c_int)), np_.ctypes.data_as(C.POINTER(C.c_int)),
+                                  len(nn), int(budget_bytes), nc.ctypes.data_as(C.POINTER(C.c_int)),
+                                  pc.ctypes.data_as(C.POINTER(C.c_int))))
+    return nc, pc
 
 
 def test_intersect(scene, rays, anyhit=False, device=False):

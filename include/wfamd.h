@@ -205,6 +205,11 @@ int wf_test_slab(const float* cases, int n, int* out, int device);
 /* Quotients n[i] / d by the multiply-high division by an invariant divisor that the persistent
  * engine uses for new-path indexing (no reference counterpart; checked against plain division). */
 int wf_test_divu32(const uint32_t* n, int count, uint32_t d, uint32_t* q, int device);
+/* The persistent engine's LDS plan for n (<= 8) scenes of n_nodes[k] nodes and n_prims[k] primitives
+ * within budget_bytes: nodes and primitive records each scene keeps in LDS (no reference
+ * counterpart; host only). */
+int wf_test_lds_plan(const int* n_nodes, const int* n_prims, int n, int64_t budget_bytes, int* node_cnt,
+                     int* prim_cnt);
 /* Radiance of single paths (px, py, sample triples) with the shared estimator, and per-path ray
  * counts (extension, shadow). */
 int wf_test_trace_paths(const wf_scene* scene, const int* pxys, int n, uint32_t seed, int max_depth, float* L,
