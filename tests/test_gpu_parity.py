@@ -343,14 +343,18 @@ def test_serving_results_equal_alone_and_in_mix(wf, scenes):
     srv.close()
 
 
-def test_batch_job_views_equal_standalone_view_renders(wf, scenes):
+@pytest.mark.parametrize("lds_kib", [0, 24])
+def test_batch_job_views_equal_standalone_view_renders(wf, scenes, lds_kib):
     # Jobs with their own image size and orbit camera over shared scenes (one BVH per scene): each
     # equals a standalone render of the same view (Scene.view copies geometry, sets the camera).
+    # At 24 KiB the LDS plan keeps sphere_grid's whole tree but not its primitive records (the
+    # all-LDS visit with leaf primitives from memory) and only heightfield's top.
+    print(f"[batch views] lds_kib {lds_kib}", flush=True)
     base = {n: scenes(n, 64, 36) for n in ("cornell555", "sphere_grid", "heightfield")}
     views = [("sphere_grid", 40, 30, 30.0, 2), ("sphere_grid", 64, 36, 200.0, 1), ("heightfield", 33, 17, 95.5, 2),
              ("cornell555", 24, 40, None, 3), ("heightfield", 50, 20, 359.0, 1), ("sphere_grid", 17, 9, None, 4)]
     specs = [(base[n], spp, 7 + k, 8, w, h, yaw) for k, (n, w, h, yaw, spp) in enumerate(views)]
-    outs, st = wf.batch_render(specs, wf.params(pool_paths=2048))
+    outs, st = wf.batch_render(specs, wf.params(pool_paths=2048, lds_kib=lds_kib))
     assert st.queue_overflows == 0 and st.invalid_hits == 0
     total = 0
     for (n, w, h, yaw, spp), spec, (rgb, var) in zip(views, specs, outs):
