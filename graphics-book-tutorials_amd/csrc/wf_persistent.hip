@@ -137,18 +137,10 @@ __device__ __forceinline__ void stage_job0(const PCtx& c, JobDev* dst) {
     for (int i = threadIdx.x; i < (int)(sizeof(JobDev) / sizeof(int)); i += blockDim.x) d[i] = src[i];
 }
 
-// The path of work index w: its job, its sample-buffer index (sample-major, then pixel) and its
-// initial state (include/wf/path_core.hpp:59-82). t: the multi-job kernels' LDS tables; job0: the
-// single-job kernels' LDS copy of job 0.
-template <bool SINGLE>
-__device__ __forceinline__ PathState new_path(const PCtx& c, const JobTab* t, const JobDev* job0, uint32_t n_jobs,
-                                              unsigned long long w, uint32_t* jid_out, uint32_t* sidx_out) {
-    uint32_t jid = 0;
-    if constexpr (!SINGLE) {
-        if (t->staged) jid = (uint32_t)find_job([&](int k) { return t->work_end[k]; }, n_jobs, w);
-        else jid = (uint32_t)find_job([&](int k) { return c.jobs[k].work_end; }, n_jobs, w);
-    }
-    const JobDev& job = SINGLE ? *job0 : c.jobs[jid];
+// The path of work index w of job `job` (index jid): its sample-buffer index (sample-major, then
+// pixel) and its initial state (include/wf/path_core.hpp:59-82).
+__device__ __forceinline__ PathState path_of_job(const JobDev& job, uint32_t jid, unsigned long long w,
+                                                 uint32_t* jid_out, uint32_t* sidx_out) {
     const uint32_t local = (uint32_t)(w - job.work_begin);
     const uint32_t W = (uint32_t)job.width, WH = W * (uint32_t)job.height;
     const uint32_t q = divu32(local, job.div_pixels);
@@ -157,6 +149,47 @@ __device__ __forceinline__ PathState new_path(const PCtx& c, const JobTab* t, co
     *jid_out = jid;
     *sidx_out = q * WH + py * W + px;
     return init_path_view(job.cam, job.width, job.height, (int)px, (int)py, job.first_sample + q, job.seed);
+}
+
+typedef __attribute__((address_space(4))) const JobDev const_job;
+
+// The fields of job j that a new path needs, read through the constant address space: with j
+// wave-uniform these are scalar loads.
+__device__ __forceinline__ JobDev job_fields_scalar(const JobDev* jobs, uint32_t j) {
+    const_job* p = (const_job*)jobs + j;
+    JobDev d{};
+    d.work_begin = p->work_begin;
+    d.width = p->width;
+    d.height = p->height;
+    d.div_pixels = DivU32{p->div_pixels.m, p->div_pixels.shift, p->div_pixels.d, 0u};
+    d.div_band = DivU32{p->div_band.m, p->div_band.shift, p->div_band.d, 0u};
+    d.cam.origin = Vec3{p->cam.origin.x, p->cam.origin.y, p->cam.origin.z};
+    d.cam.lower_left = Vec3{p->cam.lower_left.x, p->cam.lower_left.y, p->cam.lower_left.z};
+    d.cam.horizontal = Vec3{p->cam.horizontal.x, p->cam.horizontal.y, p->cam.horizontal.z};
+    d.cam.vertical = Vec3{p->cam.vertical.x, p->cam.vertical.y, p->cam.vertical.z};
+    d.first_sample = p->first_sample;
+    d.seed = p->seed;
+    return d;
+}
+
+// The path of work index w: its job, its sample-buffer index and its initial state. t: the
+// multi-job kernels' LDS tables; job0: the single-job kernels' LDS copy of job 0. In the multi-job
+// kernels the lanes that start a path together nearly always belong to one job: its record then
+// comes through scalar loads (the constant address space: job records are read-only in these
+// kernels) instead of a round trip to memory in every lane.
+template <bool SINGLE>
+__device__ __forceinline__ PathState new_path(const PCtx& c, const JobTab* t, const JobDev* job0, uint32_t n_jobs,
+                                              unsigned long long w, uint32_t* jid_out, uint32_t* sidx_out) {
+    if constexpr (SINGLE) {
+        return path_of_job(*job0, 0u, w, jid_out, sidx_out);
+    } else {
+        uint32_t jid;
+        if (t->staged) jid = (uint32_t)find_job([&](int k) { return t->work_end[k]; }, n_jobs, w);
+        else jid = (uint32_t)find_job([&](int k) { return c.jobs[k].work_end; }, n_jobs, w);
+        const uint32_t j0 = __builtin_amdgcn_readfirstlane(jid);
+        if (__ballot(jid != j0) == 0) return path_of_job(job_fields_scalar(c.jobs, j0), j0, w, jid_out, sidx_out);
+        return path_of_job(c.jobs[jid], jid, w, jid_out, sidx_out);
+    }
 }
 
 // Stages the LDS layout (BVH tops and primitive records); the whole block calls it.
