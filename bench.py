@@ -5,7 +5,7 @@ sky-gradient emitter used for next-event estimation) at 1280x720, 64 spp, depth 
 synthetic procedural input. One step = one full render of the workload (reset + all 64 passes),
 scene and film resident in HBM; the download of the image is outside the timed region.
 
-Prints one JSON line (the driver's contract) with:
+Prints one JSON line with:
   value        (extension + shadow rays of all K steps) / wall time of the K steps, in Mrays/s
   p50/p99      per-step render latency (HIP events around each step)
   roofline     dominant kernel of the flagship arm: algorithmic bytes per launch / average launch
@@ -119,8 +119,8 @@ def main():
                     help="only run the plain-copy calibration kernel over BYTES (traffic-counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (the path shards by pixel tiles; see DESIGN.md); "
-              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        print(f"bench.py measures one GPU only (across GPUs the work would shard by jobs and pixel tiles, "
+              f"one process per GPU); --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import wfamd

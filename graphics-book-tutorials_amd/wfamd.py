@@ -1,5 +1,5 @@
 """ctypes binding of libwfamd.so (include/wfamd.h) for Python callers: tests, bench.py and the
-graft entry. Plumbing only -- the engine is the native library. Loading fails loudly when the
+build/smoke entry points. Plumbing only -- the engine is the native library. Loading fails loudly when the
 library has not been built; there is no fallback path.
 
 Mirrors the reference API: Scene/make_scene (src/scenes.cpp:29), GpuRenderer

@@ -1,6 +1,6 @@
 // C entry points over the reference's own host code, compiled from its sources where they lie
-// (see oracle/Makefile, target `ref`). Used only in this container to pin the oracle and to
-// write golden fixtures; the reference code never enters the repository or the GPU box.
+// (see oracle/Makefile, target `ref`). Used only where those sources exist, to pin the oracle and
+// to write golden fixtures; the reference code itself never enters the repository.
 #include <cstring>
 #include <string>
 #include <vector>
