@@ -236,7 +236,10 @@ def main():
         "data": "synthetic: procedural sphere_grid scene (1024 jittered spheres, seeded), no external data",
         "config": {"workload": f"{args.scene} {args.width}x{args.height} {spp}spp depth {depth} seed {seed} "
                                f"(material-divergent scene)", "arm": "persistent batched wavefront",
-                   **{k: getattr(tuning, k) for k in TUNING_KEYS}, "parallelism": "single GPU"},
+                   **{k: getattr(tuning, k) for k in TUNING_KEYS},
+                   "pool_paths_used": int(tuning.pool_paths) if tuning.pool_paths > 0 else
+                   min(1 << 26, (args.width * args.height * spp + 1023) // 1024 * 1024),
+                   "parallelism": "single GPU"},
         "p50_ms": percentile(lat, 0.5), "p99_ms": percentile(lat, 0.99),
         "extension_mrays_per_s": ext / elapsed / 1e6, "rays_per_render": rays_per_render,
         "download_ms": download_ms,

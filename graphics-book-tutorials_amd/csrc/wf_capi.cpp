@@ -820,6 +820,13 @@ int wf_server_create(uint64_t budget_bytes, const wf_render_params* tuning, wf_s
         // launch, and 8 pool iterations per launch cut the serving latency (p50 57 -> 52 ms, p99
         // 136 -> 130 ms at 75% load) at the same capacity; 4 cut it further at 3% of capacity.
         if (s->tune.graph_iters <= 0) s->tune.graph_iters = 8;
+        // Pool: the largest power of two up to 2^26 slots that takes at most half the budget (the
+        // rest is for requests' films and sample buffers), at least 2^20.
+        if (s->tune.pool_paths <= 0) {
+            uint64_t p = 1ull << 26;
+            while (p > (1ull << 20) && p * wf::gpu::PersistentEngine::kBytesPerSlot > budget_bytes / 2) p >>= 1;
+            s->tune.pool_paths = (int)p;
+        }
         // Sample buffers and films of requests come from the device's stream-ordered pool; keep
         // freed memory there for the next requests instead of returning it at every synchronisation.
         int dev = 0;

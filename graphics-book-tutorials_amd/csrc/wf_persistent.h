@@ -56,7 +56,7 @@ struct PersistentJob {
 
 struct PersistentTuning {
     int block_size = 0;    // threads per workgroup: 256, 512 or 1024 (0 -> 512)
-    int pool_paths = 0;    // path slots = queue capacity (0 -> 2^22)
+    int pool_paths = 0;    // path slots = queue capacity (0 -> the render's work, up to 2^26)
     int lds_kib = 0;       // LDS per workgroup for BVH nodes and primitive records (0 -> 64 KiB; at most the
                            // device limit less the kernels' own LDS, under 1 KiB)
     int graph_iters = 0;   // iterations per captured graph (0 -> 16; rounded up to even)
@@ -169,6 +169,8 @@ public:
     void finish_job(int job, hipStream_t stream = nullptr);
     void reset_jobs();
     uint64_t pool_bytes() const { return pool_bytes_; }
+    // Device bytes of one path slot: two 64-byte survivor states, the hit and the shadow-queue entry.
+    static constexpr uint64_t kBytesPerSlot = 2 * 64 + 8 + 48;
     uint64_t bytes() const;
     int pool_paths() const { return cap_; }
 

@@ -16,7 +16,12 @@ namespace wf::gpu {
 
 namespace {
 
-constexpr int kDefaultPool = 1 << 22;
+// Path slots when the caller gives none: as many as the admitted work, up to 2^26 (12 GB of path
+// state, 184 B a slot). More slots mean fewer, fuller iterations: each iteration's launches pay
+// their prologues and the straggling waves at their end, so a render at a pool of its whole work
+// runs in depth + 1 iterations (sphere_grid 1280x720x64: 41 iterations at 2^22 slots, 19.1 ms;
+// 9 at 2^26, 14.4 ms).
+constexpr int kDefaultPool = 1 << 26;
 constexpr int kDefaultBlock = 512;
 constexpr int kDefaultGraphIters = 16;
 constexpr int kDefaultLdsKiB = 64;
@@ -1374,18 +1379,17 @@ PersistentTuning autotune_persistent(const PersistentJob* jobs, int n_jobs, hipS
             best = t;
         }
     };
-    // Coordinate search. Stage 1: workgroup size at the largest pool and LDS budget.
-    const int pools[] = {1 << 20, 1 << 21, 1 << 22};
-    int top = pools[0];
-    for (int p : pools)
-        if ((uint64_t)p <= 2 * work) top = p;
+    // Coordinate search. The pool is not searched: it is sized to the work of the render (up to
+    // kDefaultPool), which the short probe renders cannot show, and more slots were faster at
+    // every size measured. Stage 1: workgroup size.
+    (void)work;
     for (int bs : {256, 512, 1024}) {
         PersistentTuning t;
         t.block_size = bs;
         t.refill = kDefaultRefill;
         t.leaf_batch = kDefaultLeafBatch;
         t.dir_bins = kDefaultDirBins;
-        t.pool_paths = top;
+        t.pool_paths = 0;
         t.lds_kib = 64;
         time(t);
     }
@@ -1413,17 +1417,13 @@ PersistentTuning autotune_persistent(const PersistentJob* jobs, int n_jobs, hipS
         t.dir_bins = nb;
         time(t);
     }
-    // Stage 3: pool size (queue capacity) and LDS budget.
+    // Stage 3: LDS budget.
     base = best;
-    for (int p : pools) {
-        if (p > top) continue;
-        for (int kib : {32, 40, 48, 64, 128}) {
-            if (p == base.pool_paths && kib == base.lds_kib) continue;
-            PersistentTuning t = base;
-            t.pool_paths = p;
-            t.lds_kib = kib;
-            time(t);
-        }
+    for (int kib : {32, 40, 48, 64, 128}) {
+        if (kib == base.lds_kib) continue;
+        PersistentTuning t = base;
+        t.lds_kib = kib;
+        time(t);
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
