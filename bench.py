@@ -26,33 +26,14 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 METRIC = "Mrays/s at 1280x720x64spp depth 8 per scene; p50/p99 render latency"
 TUNING_KEYS = ("block_size", "pool_paths", "lds_kib", "refill", "leaf_batch", "dir_bins", "shade_block")
 HBM_PEAK_GBS = 8000.0
-# Device-memory bytes per launch with work measured by rocprofv3 FETCH_SIZE and WRITE_SIZE on this
+# Device-memory traffic against the algorithmic bytes, from rocprofv3 FETCH_SIZE and WRITE_SIZE on this
 # workload (sphere_grid 1280x720x64, default tuning), each counter scaled by what it reads on a plain
 # copy kernel of known size (FETCH_SIZE x2.0, WRITE_SIZE x1.0); source and commands:
-# profiles/persistent_sphere_grid_1280x720x64_counters.md. Valid for that workload only.
-MEASURED_TRAFFIC = {"sphere_grid 1280x720x64": {"extend": 96.5e6, "shade": 400.7e6, "shadow": 56.8e6}}
-# Wave-level VALU instructions per render (SQ_INSTS_VALU of a counter pass over 2 renders, halved) on
-# this workload, same source; taken on a build that issued slightly more instructions than the
-# current one (it still divided twice per tested sphere), so the issue fraction below is an upper
-# bound.
-MEASURED_VALU_PER_RENDER = {"sphere_grid 1280x720x64": {"extend": 5.75e9, "shade": 2.18e9, "shadow": 1.165e9}}
-# VALU issue peak of the device: one wave64 FP32 instruction per 4 cycles on each 16-lane SIMD.
-VALU_SIMDS, VALU_CLOCK_GHZ = 1024, 2.4
-# VALU lane utilisation of the kernels (same source, counter pass after the signed slab test).
-MEASURED_LANE_UTIL = {"sphere_grid 1280x720x64": {"extend": 0.534, "shadow": 0.479, "shade": 0.515,
-                                                   "megakernel": 0.193}}
-
-
-def valu_issue(instr_per_render, stage_ms):
-    """The kernel's VALU issue rate against the device's: wave-level VALU instructions per render
-    over the stage's time per render, against VALU_SIMDS SIMDs issuing one wave64 instruction per
-    4 cycles."""
-    if not instr_per_render or stage_ms <= 0:
-        return None
-    achieved = instr_per_render / (stage_ms / 1e3)
-    peak = VALU_SIMDS * VALU_CLOCK_GHZ * 1e9 / 4.0
-    return {"achieved": achieved, "peak": peak, "unit": "wave-instr/s", "frac": achieved / peak,
-            "note": "instruction count from a counter pass on an earlier build; an upper bound"}
+# profiles/persistent_sphere_grid_1280x720x64_counters.md. Those passes ran on an earlier build whose
+# path pool held 2^22 slots, so their bytes per launch do not describe this build's launches (a launch
+# now covers up to 2^26 slots); only the measured / algorithmic ratio per kernel carries over, and
+# `traffic` is reported as null.
+EARLIER_TRAFFIC_RATIO = {"sphere_grid 1280x720x64": {"extend": 1.08, "shade": 1.03, "shadow": 3.06}}
 
 
 def percentile(v, p):
@@ -205,13 +186,13 @@ def main():
         avg_s = d["ms"] / 1e3 / launches
         achieved = d["bytes"] / launches / avg_s / 1e9
         wl = f"{args.scene} {args.width}x{args.height}x{spp}"
-        traffic = MEASURED_TRAFFIC.get(wl, {}).get(dom)
+        ratio = EARLIER_TRAFFIC_RATIO.get(wl, {}).get(dom)
         roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                "note": ("extend and shadow are BVH traversal: bound by VALU issue of their node visits and "
-                         "leaf tests (valu_issue) at the lane utilisation given in lane_util, not by HBM"),
-                "lane_util": MEASURED_LANE_UTIL.get(wl),
-                "valu_issue": valu_issue(MEASURED_VALU_PER_RENDER.get(wl, {}).get(dom), d["ms"]),
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "note": ("extend and shadow are BVH traversal: bound by the issue of their node visits and "
+                         "leaf tests (LDS latency, lane utilisation), not by HBM; traffic null: no counter "
+                         "pass on this build"),
+                "traffic_over_algorithmic_earlier_build": ratio,
                 "avg_launch_us": avg_s * 1e6, "bytes_per_launch": d["bytes"] / launches,
                 "items_per_launch": d["items"] / launches,
                 "stages": {k: {"ms": v["ms"], "launches": v["launches"], "items": v["items"],
