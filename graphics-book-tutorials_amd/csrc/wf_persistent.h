@@ -59,7 +59,8 @@ struct PersistentTuning {
     int pool_paths = 0;    // path slots = queue capacity (0 -> the render's work, up to 2^26)
     int lds_kib = 0;       // LDS per workgroup for BVH nodes and primitive records (0 -> 64 KiB; at most the
                            // device limit less the kernels' own LDS, under 1 KiB)
-    int graph_iters = 0;   // iterations per captured graph (0 -> 16; rounded up to even)
+    int graph_iters = 0;   // iterations per captured graph (0 -> max_depth + 2 when every sample fits
+                           // the pool, else 16; rounded up to even)
     int refill = 0;        // traversal refill threshold in lanes of 64 (0 -> 32)
     int leaf_batch = 0;    // parked leaves that trigger a wave's leaf-test round (0 -> 16)
     int dir_bins = 0;      // survivor grouping per block round: 1 none, 8 direction octants (0 -> 1)

@@ -56,8 +56,9 @@ typedef struct wf_render_params {
     int leaf_batch;       /* persistent: parked leaves that start a wave's leaf-test round */
     int dir_bins;         /* persistent: survivor grouping, 1 none or 8 direction octants */
     int shade_block;      /* persistent: threads per shade workgroup = compaction round (128..1024) */
-    int graph_iters;      /* persistent: pool iterations per captured graph launch (0 = 16, or 8 in the
-                           * serving loop; rounded up to even) */
+    int graph_iters;      /* persistent: pool iterations per captured graph launch (0 = max_depth + 2
+                           * when every sample fits the pool, else 16; 8 in the serving loop;
+                           * rounded up to even) */
 } wf_render_params;
 
 /* RenderStats, include/wf/renderer.hpp:23-37. stage_ms: extend, classify, shade_lambert,
@@ -253,8 +254,10 @@ typedef struct wf_batch_stats {
 } wf_batch_stats;
 
 int wf_batch_render(const wf_job* jobs, int n_jobs, const wf_render_params* tuning, wf_batch_stats* stats);
-/* Autotuner: times short renders of the batch (probe_spp samples per job) over block size, pool
- * size (= queue capacity) and LDS budget; writes the fastest into *tuning. */
+/* Autotuner: a coordinate search over short renders of the batch (probe_spp samples per job):
+ * block size, refill threshold x leaf batch, shade block, survivor grouping, LDS budget. The three
+ * fastest candidates and the default tuning are then timed on the whole batch, and the fastest of
+ * them is written into *tuning (pool_paths 0: sized to the work). Renders into the jobs' films. */
 int wf_autotune(const wf_job* jobs, int n_jobs, int probe_spp, wf_render_params* tuning);
 
 /* ---- serving loop: continuous batching under a device-memory budget (no reference counterpart) ---- */
