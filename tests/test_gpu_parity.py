@@ -139,6 +139,20 @@ def test_persistent_equals_megakernel(wf, scenes, name, w, h, pool):
     assert st.queue_overflows == 0 and st.invalid_hits == 0
 
 
+@pytest.mark.parametrize("depth", [1, 8, 12])
+@pytest.mark.parametrize("graph_iters,pool", [(0, 0), (2, 0), (3, 0), (0, 2048), (6, 2048)])
+def test_persistent_graph_length_and_depth(wf, scenes, depth, graph_iters, pool):
+    """Graphs shorter than a render (several launches), the default length (max_depth + 2 when the
+    pool holds every sample, else 16) and depths beyond 8 all give the megakernel's bits."""
+    s = scenes("divergence", 40, 24)
+    mk, mst = gpu_sums(wf, s, "megakernel", 3, depth=depth)
+    pe, st = gpu_sums(wf, s, "persistent", 3, depth=depth, graph_iters=graph_iters, pool_paths=pool)
+    for a, b, k in zip(pe, mk, ("rgb", "lum", "lum2")):
+        assert_bitwise(a, b, f"depth {depth} graph_iters {graph_iters} pool {pool} {k}")
+    assert st.extension_rays == mst.extension_rays and st.shadow_rays == mst.shadow_rays
+    assert st.queue_overflows == 0 and st.invalid_hits == 0
+
+
 @pytest.mark.parametrize("w,h", [(1, 1), (33, 17), (1, 720), (31, 1)])
 def test_odd_sizes_all_arms_agree(wf, scenes, w, h):
     s = scenes("divergence", w, h)
