@@ -196,7 +196,10 @@ def main():
                 "avg_launch_us": avg_s * 1e6, "bytes_per_launch": d["bytes"] / launches,
                 "items_per_launch": d["items"] / launches,
                 "stages": {k: {"ms": v["ms"], "launches": v["launches"], "items": v["items"],
-                               "gbs": v["bytes"] / (v["ms"] / 1e3) / 1e9 if v["ms"] > 0 else None}
+                               "gbs": v["bytes"] / (v["ms"] / 1e3) / 1e9 if v["ms"] > 0 else None,
+                               # occupancy: waves of the persistent grid resident on each SIMD
+                               # (4 SIMDs per CU; the device allows at most 8 per SIMD)
+                               "waves_per_simd": v["threads"] / 64 / (4 * dev["compute_units"])}
                            for k, v in stages.items()}}
     r.close()
 

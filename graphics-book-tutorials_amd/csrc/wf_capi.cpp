@@ -100,6 +100,7 @@ void to_stats(const wf::RenderStats& s, wf_render_stats* o) {
         o->stage_items[k] = s.stages.items[k];
         o->stage_launches[k] = s.stages.launches[k];
         o->stage_bytes[k] = s.stages.bytes[k];
+        o->stage_threads[k] = s.stages.threads[k];
     }
 }
 
@@ -741,6 +742,7 @@ extern "C" int wf_batch_render(const wf_job* jobs, int n_jobs, const wf_render_p
                 stats->stage_items[k] = stages.items[k];
                 stats->stage_launches[k] = stages.launches[k];
                 stats->stage_bytes[k] = stages.bytes[k];
+                stats->stage_threads[k] = stages.threads[k];
             }
         }
         return (int)WF_OK;

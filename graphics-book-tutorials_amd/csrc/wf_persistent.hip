@@ -1195,6 +1195,9 @@ void PersistentEngine::run_profiled(StageTimes* out) {
         out->launches[0] += 1;
         out->launches[2] += 1;
         out->launches[5] += 1;
+        out->threads[0] = (long long)L.gx * m.block;
+        out->threads[2] = (long long)L.gs * m.shade_block;
+        out->threads[5] = (long long)L.gw * m.block;
         out->launches[7] += 1;
     }
     // The captured graph runs an even number of iterations from buffer 0; the work has drained,

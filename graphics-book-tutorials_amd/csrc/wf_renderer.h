@@ -41,6 +41,8 @@ struct StageTimes {   // milliseconds summed over a render
     long long launches[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     // Algorithmic device-memory bytes per stage (persistent engine; 0 where not computed).
     long long bytes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // Threads per launch of each stage (persistent engine: persistent grid x workgroup size).
+    long long threads[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 struct RenderStats {

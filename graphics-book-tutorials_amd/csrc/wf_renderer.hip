@@ -183,6 +183,7 @@ void GpuRenderer::run_passes(int first_sample, int count, const RenderParams& p,
             stats->stages.items[k] += st.items[k];
             stats->stages.launches[k] += st.launches[k];
             stats->stages.bytes[k] += st.bytes[k];
+            stats->stages.threads[k] = std::max(stats->stages.threads[k], st.threads[k]);
         }
     }
 }

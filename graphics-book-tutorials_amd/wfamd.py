@@ -42,19 +42,19 @@ class RenderStats(C.Structure):
                 ("shadow_rays", C.c_uint64), ("queue_high_water", C.c_int64 * 5), ("queue_overflows", C.c_int64),
                 ("invalid_hits", C.c_int64), ("peak_vram_bytes", C.c_uint64), ("stage_ms", C.c_double * 8),
                 ("iterations", C.c_int64), ("stage_items", C.c_int64 * 8), ("stage_launches", C.c_int64 * 8),
-                ("stage_bytes", C.c_int64 * 8)]
+                ("stage_bytes", C.c_int64 * 8), ("stage_threads", C.c_int64 * 8)]
 
     STAGES = ("extend", "classify", "shade_lambert", "shade_metal", "shade_dielectric", "shadow", "compact", "other")
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
-        for k in ("queue_high_water", "stage_ms", "stage_items", "stage_launches", "stage_bytes"):
+        for k in ("queue_high_water", "stage_ms", "stage_items", "stage_launches", "stage_bytes", "stage_threads"):
             d[k] = list(getattr(self, k))
         return d
 
     def stages(self):
         return {n: {"ms": self.stage_ms[i], "items": self.stage_items[i], "launches": self.stage_launches[i],
-                    "bytes": self.stage_bytes[i]}
+                    "bytes": self.stage_bytes[i], "threads": self.stage_threads[i]}
                 for i, n in enumerate(self.STAGES) if self.stage_launches[i] > 0}
 
 
@@ -97,11 +97,11 @@ class BatchStats(C.Structure):
                 ("lds_kib", C.c_int), ("refill", C.c_int), ("leaf_batch", C.c_int),
                 ("dir_bins", C.c_int), ("shade_block", C.c_int), ("device_bytes", C.c_uint64),
                 ("stage_ms", C.c_double * 8), ("stage_items", C.c_int64 * 8), ("stage_launches", C.c_int64 * 8),
-                ("stage_bytes", C.c_int64 * 8)]
+                ("stage_bytes", C.c_int64 * 8), ("stage_threads", C.c_int64 * 8)]
 
     def stages(self):
         return {n: {"ms": self.stage_ms[i], "items": self.stage_items[i], "launches": self.stage_launches[i],
-                    "bytes": self.stage_bytes[i]}
+                    "bytes": self.stage_bytes[i], "threads": self.stage_threads[i]}
                 for i, n in enumerate(RenderStats.STAGES) if self.stage_launches[i] > 0}
 
 

@@ -80,6 +80,8 @@ typedef struct wf_render_stats {
     int64_t stage_items[8];
     int64_t stage_launches[8];
     int64_t stage_bytes[8];      /* persistent engine: algorithmic device-memory bytes per stage */
+    int64_t stage_threads[8];    /* persistent engine: threads per launch of each stage (persistent
+                                  * grid x workgroup size: the occupancy the kernel runs at) */
 } wf_render_stats;
 
 typedef struct wf_scene_info {
@@ -251,6 +253,7 @@ typedef struct wf_batch_stats {
     int64_t stage_items[8];
     int64_t stage_launches[8];
     int64_t stage_bytes[8];
+    int64_t stage_threads[8];
 } wf_batch_stats;
 
 int wf_batch_render(const wf_job* jobs, int n_jobs, const wf_render_params* tuning, wf_batch_stats* stats);
