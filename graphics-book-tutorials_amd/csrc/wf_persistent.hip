@@ -1305,7 +1305,12 @@ void PersistentEngine::render(const PersistentJob* jobs, int n_jobs, const Persi
     // No more slots than paths in the batch.
     PersistentTuning tt = t;
     const uint64_t want = tt.pool_paths > 0 ? (uint64_t)tt.pool_paths : (uint64_t)kDefaultPool;
-    tt.pool_paths = (int)std::min<uint64_t>(want, std::max<uint64_t>(1024, (total + 1023) & ~1023ull));
+    const uint64_t need = std::min<uint64_t>(want, std::max<uint64_t>(1024, (total + 1023) & ~1023ull));
+    // A pool sized to the work is kept while later renders need no more: results do not depend on
+    // the pool size, and a new pool would mean freeing and allocating up to 12 GB, uploading the
+    // scene table again and capturing a new graph.
+    const bool keep = t.pool_paths == 0 && !m_->pool_allocs.empty() && (uint64_t)cap_ >= need && cap_ <= kDefaultPool;
+    tt.pool_paths = keep ? cap_ : (int)need;
     // When every sample fits the pool, all paths start in the first iteration and have ended after
     // max_depth + 1; the plan of the next iteration finds no work and marks the render done. A
     // graph of max_depth + 2 iterations then finishes in one launch without idle iterations (each
