@@ -439,9 +439,7 @@ int wf_renderer_download(wf_renderer* r, int spp_done, float* rgb, float* lum_va
     return guard([&] {
         if (!r) throw std::invalid_argument("null renderer");
         if (spp_done < 1) throw std::invalid_argument("spp_done must be >= 1");
-        wf::RenderResult res = r->r->download(spp_done);
-        if (rgb) std::memcpy(rgb, res.rgb.data(), res.rgb.size() * sizeof(float));
-        if (lum_var) std::memcpy(lum_var, res.lum_var.data(), res.lum_var.size() * sizeof(float));
+        r->r->download_to(spp_done, rgb, lum_var);
         return (int)WF_OK;
     });
 }

@@ -76,6 +76,8 @@ public:
     void reset();
     void run_passes(int first_sample, int count, const RenderParams& p, RenderStats* stats);
     RenderResult download(int spp_done);
+    // download() into caller-owned buffers (3 * pixels and pixels floats; either may be null).
+    void download_to(int spp_done, float* rgb, float* lum_var);
     void download_mean(int spp_done, std::vector<float>* rgb);
     void download_sums(std::vector<float>* rgb, std::vector<float>* lum, std::vector<float>* lum2);
     uint64_t device_bytes() const;

@@ -46,6 +46,9 @@ struct GpuRenderer::Impl {
     int width = 0, height = 0, pixels = 0;
     gpu::DeviceScene scene;
     gpu::DeviceFilm film{};
+    // Pinned host copy of the film sums (rgb 3P, lum P, lum2 P floats), allocated at the first
+    // download: the copies then run at DMA speed instead of through pageable staging.
+    float* pinned = nullptr;
     gpu::DevCounters* d_ctr = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -61,6 +64,7 @@ struct GpuRenderer::Impl {
         if (film.lum) (void)hipFree(film.lum);
         if (film.lum2) (void)hipFree(film.lum2);
         if (d_ctr) (void)hipFree(d_ctr);
+        if (pinned) (void)hipHostFree(pinned);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -188,43 +192,65 @@ void GpuRenderer::run_passes(int first_sample, int count, const RenderParams& p,
     }
 }
 
+namespace {
+
+// Copies the requested film sums into the renderer's pinned buffer (rgb at 0, lum at 3P, lum2 at
+// 4P floats) on its stream and waits for them.
+const float* stage_film(GpuRenderer::Impl& m, bool rgb, bool lum, bool lum2) {
+    const size_t P = (size_t)m.pixels;
+    if (!m.pinned) gpu::check(hipHostMalloc((void**)&m.pinned, 5 * P * sizeof(float), hipHostMallocDefault), "hipHostMalloc film");
+    if (rgb)
+        gpu::check(hipMemcpyAsync(m.pinned, m.film.rgb, 3 * P * sizeof(float), hipMemcpyDeviceToHost, m.stream),
+                   "download rgb");
+    if (lum)
+        gpu::check(hipMemcpyAsync(m.pinned + 3 * P, m.film.lum, P * sizeof(float), hipMemcpyDeviceToHost, m.stream),
+                   "download lum");
+    if (lum2)
+        gpu::check(hipMemcpyAsync(m.pinned + 4 * P, m.film.lum2, P * sizeof(float), hipMemcpyDeviceToHost, m.stream),
+                   "download lum2");
+    gpu::check(hipStreamSynchronize(m.stream), "download sync");
+    return m.pinned;
+}
+
+}  // namespace
+
 void GpuRenderer::download_sums(std::vector<float>* rgb, std::vector<float>* lum, std::vector<float>* lum2) {
     Impl& m = *impl_;
     const size_t P = (size_t)m.pixels;
-    if (rgb) {
-        rgb->resize(3 * P);
-        gpu::check(hipMemcpy(rgb->data(), m.film.rgb, 3 * P * sizeof(float), hipMemcpyDeviceToHost), "download rgb");
-    }
-    if (lum) {
-        lum->resize(P);
-        gpu::check(hipMemcpy(lum->data(), m.film.lum, P * sizeof(float), hipMemcpyDeviceToHost), "download lum");
-    }
-    if (lum2) {
-        lum2->resize(P);
-        gpu::check(hipMemcpy(lum2->data(), m.film.lum2, P * sizeof(float), hipMemcpyDeviceToHost), "download lum2");
-    }
+    const float* h = stage_film(m, rgb != nullptr, lum != nullptr, lum2 != nullptr);
+    if (rgb) rgb->assign(h, h + 3 * P);
+    if (lum) lum->assign(h + 3 * P, h + 4 * P);
+    if (lum2) lum2->assign(h + 4 * P, h + 5 * P);
 }
 
 RenderResult GpuRenderer::download(int spp_done) {
     if (spp_done < 1) throw std::runtime_error("download: spp_done must be >= 1");
-    std::vector<float> rgb, lum, lum2;
-    download_sums(&rgb, &lum, &lum2);
+    Impl& m = *impl_;
+    const size_t P = (size_t)m.pixels;
+    const float* h = stage_film(m, true, true, true);
     RenderResult r;
-    r.width = impl_->width;
-    r.height = impl_->height;
+    r.width = m.width;
+    r.height = m.height;
     r.spp = spp_done;
-    r.rgb.resize(rgb.size());
-    r.lum_var.resize(lum.size());
-    finalize_film(rgb.data(), lum.data(), lum2.data(), impl_->pixels, spp_done, r.rgb.data(), r.lum_var.data());
+    r.rgb.resize(3 * P);
+    r.lum_var.resize(P);
+    finalize_film(h, h + 3 * P, h + 4 * P, m.pixels, spp_done, r.rgb.data(), r.lum_var.data());
     return r;
+}
+
+void GpuRenderer::download_to(int spp_done, float* rgb, float* lum_var) {
+    if (spp_done < 1) throw std::runtime_error("download: spp_done must be >= 1");
+    Impl& m = *impl_;
+    const size_t P = (size_t)m.pixels;
+    const float* h = stage_film(m, rgb != nullptr, lum_var != nullptr, lum_var != nullptr);
+    finalize_film(h, h + 3 * P, h + 4 * P, m.pixels, spp_done, rgb, lum_var);
 }
 
 void GpuRenderer::download_mean(int spp_done, std::vector<float>* rgb) {
     if (spp_done < 1) throw std::runtime_error("download_mean: spp_done must be >= 1");
-    std::vector<float> sums;
-    download_sums(&sums, nullptr, nullptr);
-    rgb->resize(sums.size());
-    finalize_film(sums.data(), nullptr, nullptr, impl_->pixels, spp_done, rgb->data(), nullptr);
+    const float* h = stage_film(*impl_, true, false, false);
+    rgb->resize(3 * (size_t)impl_->pixels);
+    finalize_film(h, nullptr, nullptr, impl_->pixels, spp_done, rgb->data(), nullptr);
 }
 
 RenderResult GpuRenderer::render(const RenderParams& p, RenderStats* stats) {
