@@ -134,6 +134,25 @@ struct SceneCache {
     }
 };
 
+// Fixed pinned host buffer for film downloads, allocated once: a copy into it runs at DMA speed,
+// where a pageable destination goes through the runtime's staging. Films larger than it stream
+// through it in pixel chunks. (A buffer grown to each larger film cost the serving loop more than
+// it saved: every growth page-locks tens of MB, and freeing the old one waits for the device.)
+struct PinnedStage {
+    static constexpr size_t kFloats = size_t(4) << 20;   // 16 MiB
+    float* p = nullptr;
+    PinnedStage() = default;
+    PinnedStage(const PinnedStage&) = delete;
+    PinnedStage& operator=(const PinnedStage&) = delete;
+    ~PinnedStage() {
+        if (p) (void)hipHostFree(p);
+    }
+    float* get() {
+        if (!p) wf::gpu::check(hipHostMalloc((void**)&p, kFloats * sizeof(float), hipHostMallocDefault), "hipHostMalloc film stage");
+        return p;
+    }
+};
+
 struct DevFilmAlloc {
     wf::gpu::DeviceFilm film{};
     int pixels = 0;
@@ -147,13 +166,24 @@ struct DevFilmAlloc {
         film.pixels = px;
         wf::gpu::check(hipMemsetAsync(film.rgb, 0, (size_t)px * 5 * sizeof(float), stream), "hipMemset film");
     }
-    void download(int spp, float* rgb, float* lum_var, hipStream_t stream) {
-        std::vector<float> h((size_t)pixels * 5);
-        wf::gpu::check(hipMemcpyAsync(h.data(), film.rgb, h.size() * sizeof(float), hipMemcpyDeviceToHost, stream),
-                       "download film");
-        wf::gpu::check(hipStreamSynchronize(stream), "download sync");
-        wf::finalize_film(h.data(), h.data() + 3 * (size_t)pixels, h.data() + 4 * (size_t)pixels, pixels, spp, rgb,
-                          lum_var);
+    void download(int spp, float* rgb, float* lum_var, hipStream_t stream, PinnedStage& stage) {
+        float* h = stage.get();
+        const size_t P = (size_t)pixels, chunk = PinnedStage::kFloats / 5;
+        for (size_t a = 0; a < P; a += chunk) {
+            const size_t n = std::min(chunk, P - a);
+            if (rgb)
+                wf::gpu::check(hipMemcpyAsync(h, film.rgb + 3 * a, 3 * n * sizeof(float), hipMemcpyDeviceToHost, stream),
+                               "download film");
+            if (lum_var) {
+                wf::gpu::check(hipMemcpyAsync(h + 3 * n, film.lum + a, n * sizeof(float), hipMemcpyDeviceToHost, stream),
+                               "download film");
+                wf::gpu::check(hipMemcpyAsync(h + 4 * n, film.lum2 + a, n * sizeof(float), hipMemcpyDeviceToHost, stream),
+                               "download film");
+            }
+            wf::gpu::check(hipStreamSynchronize(stream), "download sync");
+            wf::finalize_film(h, h + 3 * n, h + 4 * n, (int)n, spp, rgb ? rgb + 3 * a : nullptr,
+                              lum_var ? lum_var + a : nullptr);
+        }
     }
     void release() {
         if (film.rgb) (void)(owner ? hipFreeAsync(film.rgb, owner) : hipFree(film.rgb));
@@ -660,6 +690,7 @@ struct BatchSetup {
     SceneCache scenes;
     std::vector<DevFilmAlloc> films;
     std::vector<wf::gpu::PersistentJob> jobs;
+    PinnedStage stage;
     ~BatchSetup() {
         for (DevFilmAlloc& f : films) f.release();
         if (ctr) (void)hipFree(ctr);
@@ -713,7 +744,7 @@ extern "C" int wf_batch_render(const wf_job* jobs, int n_jobs, const wf_render_p
         wf::gpu::check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        for (int i = 0; i < n_jobs; ++i) b.films[i].download(jobs[i].spp, jobs[i].rgb, jobs[i].lum_var, b.stream);
+        for (int i = 0; i < n_jobs; ++i) b.films[i].download(jobs[i].spp, jobs[i].rgb, jobs[i].lum_var, b.stream, b.stage);
         if (stats) {
             wf::gpu::DevCounters c{};
             wf::gpu::check(hipMemcpy(&c, b.ctr, sizeof(c), hipMemcpyDeviceToHost), "copy counters");
@@ -781,6 +812,7 @@ struct wf_server {
     hipStream_t stream = nullptr;
     // Finished requests are resolved and downloaded here while the next launch runs on `stream`.
     hipStream_t side = nullptr;
+    PinnedStage stage;                // finished films are downloaded through it on `side`
     int pending = -1;                 // snapshot slot of the last launch, not yet collected
     wf::gpu::DevCounters seen{};      // counters as of the last collected snapshot
     wf::gpu::DevCounters* ctr = nullptr;
@@ -837,6 +869,7 @@ int wf_server_create(uint64_t budget_bytes, const wf_render_params* tuning, wf_s
         wf::gpu::check(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep), "hipMemPoolSetAttribute");
         wf::gpu::check(hipStreamCreate(&s->stream), "hipStreamCreate");
         wf::gpu::check(hipStreamCreate(&s->side), "hipStreamCreate");
+        s->stage.get();   // page-locked once, here, rather than inside the first step
         wf::gpu::check(hipMalloc(&s->ctr, sizeof(wf::gpu::DevCounters)), "hipMalloc counters");
         wf::gpu::check(hipMemsetAsync(s->ctr, 0, sizeof(wf::gpu::DevCounters), s->stream), "hipMemset counters");
         s->eng.open(s->tune, s->ctr, s->stream);
@@ -925,7 +958,7 @@ int wf_server_step(wf_server* s, int launches) {
                         waited = true;
                     }
                     s->eng.finish_job(r->engine_job, s->side);
-                    r->film.download(r->job.spp, r->job.rgb, r->job.lum_var, s->side);
+                    r->film.download(r->job.spp, r->job.rgb, r->job.lum_var, s->side, s->stage);
                     r->film.owner = s->side;
                     r->film.release();
                     r->state = 2;

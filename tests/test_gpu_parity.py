@@ -357,6 +357,23 @@ def test_serving_results_equal_alone_and_in_mix(wf, scenes):
     srv.close()
 
 
+def test_batch_film_larger_than_one_download_chunk(wf, scenes):
+    # A batch job's film is downloaded through a fixed pinned buffer in chunks of 838,860 pixels
+    # (16 MiB at 20 B per pixel); 1280x720 takes two, and 2 spp makes the variance nonzero.
+    print("[batch chunked download] cornell555 1280x720 2 spp", flush=True)
+    s = scenes("cornell555", 1280, 720)
+    outs, st = wf.batch_render([(s, 2, 11, 8, 1280, 720, None)], wf.params())
+    r = wf.Renderer(s.view(1280, 720, None), "megakernel")
+    want, wvar, wst = r.render(wf.params(spp=2, seed=11))
+    r.close()
+    rgb, var = outs[0]
+    assert rgb.size == 3 * 1280 * 720 and var.size == 1280 * 720
+    assert_bitwise(rgb, want, "chunked rgb")
+    assert_bitwise(var, wvar, "chunked var")
+    assert np.count_nonzero(var) > 0
+    assert st.extension_rays == wst.extension_rays and st.shadow_rays == wst.shadow_rays
+
+
 @pytest.mark.parametrize("lds_kib", [0, 24])
 def test_batch_job_views_equal_standalone_view_renders(wf, scenes, lds_kib):
     # Jobs with their own image size and orbit camera over shared scenes (one BVH per scene): each
