@@ -29,9 +29,9 @@ HBM_PEAK_GBS = 8000.0
 # Device-memory traffic against the algorithmic bytes, from rocprofv3 FETCH_SIZE and WRITE_SIZE on this
 # workload (sphere_grid 1280x720x64, default tuning), each counter scaled by what it reads on a plain
 # copy kernel of known size (FETCH_SIZE x2.0, WRITE_SIZE x1.0); source and commands:
-# profiles/persistent_sphere_grid_1280x720x64_counters.md. Those passes ran on an earlier build whose
-# path pool held 2^22 slots, so their bytes per launch do not describe this build's launches (a launch
-# now covers up to 2^26 slots); only the measured / algorithmic ratio per kernel carries over, and
+# profiles/persistent_sphere_grid_1280x720x64_counters.md. Those passes ran on an earlier version whose
+# path pool held 2^22 slots, so their bytes per launch do not describe the current launches (a
+# launch now covers up to 2^26 slots); only the measured / algorithmic ratio per kernel carries over, and
 # `traffic` is reported as null.
 EARLIER_TRAFFIC_RATIO = {"sphere_grid 1280x720x64": {"extend": 1.08, "shade": 1.03, "shadow": 3.06}}
 
@@ -191,8 +191,8 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                 "note": ("extend and shadow are BVH traversal: bound by the issue of their node visits and "
                          "leaf tests (LDS latency, lane utilisation), not by HBM; traffic null: no counter "
-                         "pass on this build"),
-                "traffic_over_algorithmic_earlier_build": ratio,
+                         "pass since the pool grew to 2^26 slots"),
+                "traffic_over_algorithmic_earlier_version": ratio,
                 "avg_launch_us": avg_s * 1e6, "bytes_per_launch": d["bytes"] / launches,
                 "items_per_launch": d["items"] / launches,
                 "stages": {k: {"ms": v["ms"], "launches": v["launches"], "items": v["items"],
