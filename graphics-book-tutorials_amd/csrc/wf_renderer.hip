@@ -49,6 +49,7 @@ struct GpuRenderer::Impl {
     // Pinned host copy of the film sums (rgb 3P, lum P, lum2 P floats), allocated at the first
     // download: the copies then run at DMA speed instead of through pageable staging.
     float* pinned = nullptr;
+    hipEvent_t dl_ev[4] = {};   // one per download range (download_to)
     gpu::DevCounters* d_ctr = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -65,6 +66,8 @@ struct GpuRenderer::Impl {
         if (film.lum2) (void)hipFree(film.lum2);
         if (d_ctr) (void)hipFree(d_ctr);
         if (pinned) (void)hipHostFree(pinned);
+        for (hipEvent_t e : dl_ev)
+            if (e) (void)hipEventDestroy(e);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -224,26 +227,46 @@ void GpuRenderer::download_sums(std::vector<float>* rgb, std::vector<float>* lum
 }
 
 RenderResult GpuRenderer::download(int spp_done) {
-    if (spp_done < 1) throw std::runtime_error("download: spp_done must be >= 1");
-    Impl& m = *impl_;
-    const size_t P = (size_t)m.pixels;
-    const float* h = stage_film(m, true, true, true);
     RenderResult r;
-    r.width = m.width;
-    r.height = m.height;
+    r.width = impl_->width;
+    r.height = impl_->height;
     r.spp = spp_done;
-    r.rgb.resize(3 * P);
-    r.lum_var.resize(P);
-    finalize_film(h, h + 3 * P, h + 4 * P, m.pixels, spp_done, r.rgb.data(), r.lum_var.data());
+    r.rgb.resize(3 * (size_t)impl_->pixels);
+    r.lum_var.resize((size_t)impl_->pixels);
+    download_to(spp_done, r.rgb.data(), r.lum_var.data());
     return r;
 }
 
+// The film sums are copied into the pinned buffer in up to four pixel ranges, with an event after
+// each; each range is finalized into the caller's buffers as soon as its copy has landed, so the
+// host finalize of one range overlaps the copies of the next.
 void GpuRenderer::download_to(int spp_done, float* rgb, float* lum_var) {
     if (spp_done < 1) throw std::runtime_error("download: spp_done must be >= 1");
     Impl& m = *impl_;
     const size_t P = (size_t)m.pixels;
-    const float* h = stage_film(m, rgb != nullptr, lum_var != nullptr, lum_var != nullptr);
-    finalize_film(h, h + 3 * P, h + 4 * P, m.pixels, spp_done, rgb, lum_var);
+    if (!m.pinned) gpu::check(hipHostMalloc((void**)&m.pinned, 5 * P * sizeof(float), hipHostMallocDefault), "hipHostMalloc film");
+    const int K = P >= (size_t(1) << 16) ? 4 : 1;
+    const size_t step = (P + K - 1) / K;
+    for (int k = 0; k < K; ++k) {
+        if (!m.dl_ev[k]) gpu::check(hipEventCreateWithFlags(&m.dl_ev[k], hipEventDisableTiming), "hipEventCreate");
+        const size_t a = std::min(P, k * step), n = std::min(step, P - a);
+        if (rgb && n)
+            gpu::check(hipMemcpyAsync(m.pinned + 3 * a, m.film.rgb + 3 * a, 3 * n * sizeof(float),
+                                      hipMemcpyDeviceToHost, m.stream), "download rgb");
+        if (lum_var && n) {
+            gpu::check(hipMemcpyAsync(m.pinned + 3 * P + a, m.film.lum + a, n * sizeof(float), hipMemcpyDeviceToHost,
+                                      m.stream), "download lum");
+            gpu::check(hipMemcpyAsync(m.pinned + 4 * P + a, m.film.lum2 + a, n * sizeof(float), hipMemcpyDeviceToHost,
+                                      m.stream), "download lum2");
+        }
+        gpu::check(hipEventRecord(m.dl_ev[k], m.stream), "hipEventRecord");
+    }
+    for (int k = 0; k < K; ++k) {
+        gpu::check(hipEventSynchronize(m.dl_ev[k]), "download sync");
+        const size_t a = std::min(P, k * step), n = std::min(step, P - a);
+        finalize_film(m.pinned + 3 * a, m.pinned + 3 * P + a, m.pinned + 4 * P + a, (int)n, spp_done,
+                      rgb ? rgb + 3 * a : nullptr, lum_var ? lum_var + a : nullptr);
+    }
 }
 
 void GpuRenderer::download_mean(int spp_done, std::vector<float>* rgb) {
