@@ -135,20 +135,26 @@ struct SceneCache {
 };
 
 // Fixed pinned host buffer for film downloads, allocated once: a copy into it runs at DMA speed,
-// where a pageable destination goes through the runtime's staging. Films larger than it stream
-// through it in pixel chunks. (A buffer grown to each larger film cost the serving loop more than
+// where a pageable destination goes through the runtime's staging. Films stream through its two
+// halves in pixel chunks: the next chunk is copied into one half while the host finalizes the
+// chunk in the other. (A buffer grown to each larger film cost the serving loop more than
 // it saved: every growth page-locks tens of MB, and freeing the old one waits for the device.)
 struct PinnedStage {
     static constexpr size_t kFloats = size_t(4) << 20;   // 16 MiB
     float* p = nullptr;
+    hipEvent_t ev[2] = {};   // copies into half 0 / half 1 have landed
     PinnedStage() = default;
     PinnedStage(const PinnedStage&) = delete;
     PinnedStage& operator=(const PinnedStage&) = delete;
     ~PinnedStage() {
         if (p) (void)hipHostFree(p);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
     }
     float* get() {
         if (!p) wf::gpu::check(hipHostMalloc((void**)&p, kFloats * sizeof(float), hipHostMallocDefault), "hipHostMalloc film stage");
+        for (hipEvent_t& e : ev)
+            if (!e) wf::gpu::check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
         return p;
     }
 };
@@ -167,9 +173,11 @@ struct DevFilmAlloc {
         wf::gpu::check(hipMemsetAsync(film.rgb, 0, (size_t)px * 5 * sizeof(float), stream), "hipMemset film");
     }
     void download(int spp, float* rgb, float* lum_var, hipStream_t stream, PinnedStage& stage) {
-        float* h = stage.get();
-        const size_t P = (size_t)pixels, chunk = PinnedStage::kFloats / 5;
-        for (size_t a = 0; a < P; a += chunk) {
+        float* base = stage.get();
+        const size_t P = (size_t)pixels, half = PinnedStage::kFloats / 2, chunk = half / 5;
+        // Copies pixels [a, a + chunk) into half `s` of the stage and marks them with ev[s].
+        auto issue = [&](size_t a, int s) {
+            float* h = base + s * half;
             const size_t n = std::min(chunk, P - a);
             if (rgb)
                 wf::gpu::check(hipMemcpyAsync(h, film.rgb + 3 * a, 3 * n * sizeof(float), hipMemcpyDeviceToHost, stream),
@@ -180,7 +188,16 @@ struct DevFilmAlloc {
                 wf::gpu::check(hipMemcpyAsync(h + 4 * n, film.lum2 + a, n * sizeof(float), hipMemcpyDeviceToHost, stream),
                                "download film");
             }
-            wf::gpu::check(hipStreamSynchronize(stream), "download sync");
+            wf::gpu::check(hipEventRecord(stage.ev[s], stream), "hipEventRecord");
+        };
+        if (P > 0) issue(0, 0);
+        int s = 0;
+        for (size_t a = 0; a < P; a += chunk, s ^= 1) {
+            // The other half was finalized in the previous step, so the next chunk may land there.
+            if (a + chunk < P) issue(a + chunk, s ^ 1);
+            wf::gpu::check(hipEventSynchronize(stage.ev[s]), "download sync");
+            const float* h = base + s * half;
+            const size_t n = std::min(chunk, P - a);
             wf::finalize_film(h, h + 3 * n, h + 4 * n, (int)n, spp, rgb ? rgb + 3 * a : nullptr,
                               lum_var ? lum_var + a : nullptr);
         }

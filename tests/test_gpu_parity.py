@@ -358,8 +358,9 @@ def test_serving_results_equal_alone_and_in_mix(wf, scenes):
 
 
 def test_batch_film_larger_than_one_download_chunk(wf, scenes):
-    # A batch job's film is downloaded through a fixed pinned buffer in chunks of 838,860 pixels
-    # (16 MiB at 20 B per pixel); 1280x720 takes two, and 2 spp makes the variance nonzero.
+    # A batch job's film is downloaded through the two halves of a fixed 16 MiB pinned buffer in
+    # chunks of 419,430 pixels (8 MiB at 20 B per pixel); 1280x720 takes three, so both halves
+    # are reused, and 2 spp makes the variance nonzero.
     print("[batch chunked download] cornell555 1280x720 2 spp", flush=True)
     s = scenes("cornell555", 1280, 720)
     outs, st = wf.batch_render([(s, 2, 11, 8, 1280, 720, None)], wf.params())
