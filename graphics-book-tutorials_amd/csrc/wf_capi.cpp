@@ -701,9 +701,28 @@ wf::gpu::PersistentTuning to_tuning(const wf_render_params* p) {
     return t;
 }
 
+// Pinned host mirror of a DevCounters block: counter reads are DMA copies into it, not pageable
+// copies through the runtime's staging buffers.
+struct PinnedCounters {
+    wf::gpu::DevCounters* h = nullptr;
+    PinnedCounters() = default;
+    PinnedCounters(const PinnedCounters&) = delete;
+    PinnedCounters& operator=(const PinnedCounters&) = delete;
+    ~PinnedCounters() {
+        if (h) (void)hipHostFree(h);
+    }
+    wf::gpu::DevCounters read(const wf::gpu::DevCounters* d, hipStream_t stream) {
+        if (!h) wf::gpu::check(hipHostMalloc((void**)&h, sizeof(*h), hipHostMallocDefault), "hipHostMalloc counters");
+        wf::gpu::check(hipMemcpyAsync(h, d, sizeof(*h), hipMemcpyDeviceToHost, stream), "copy counters");
+        wf::gpu::check(hipStreamSynchronize(stream), "counters sync");
+        return *h;
+    }
+};
+
 struct BatchSetup {
     hipStream_t stream = nullptr;
     wf::gpu::DevCounters* ctr = nullptr;
+    PinnedCounters hctr;
     SceneCache scenes;
     std::vector<DevFilmAlloc> films;
     std::vector<wf::gpu::PersistentJob> jobs;
@@ -763,8 +782,7 @@ extern "C" int wf_batch_render(const wf_job* jobs, int n_jobs, const wf_render_p
         (void)hipEventDestroy(e1);
         for (int i = 0; i < n_jobs; ++i) b.films[i].download(jobs[i].spp, jobs[i].rgb, jobs[i].lum_var, b.stream, b.stage);
         if (stats) {
-            wf::gpu::DevCounters c{};
-            wf::gpu::check(hipMemcpy(&c, b.ctr, sizeof(c), hipMemcpyDeviceToHost), "copy counters");
+            const wf::gpu::DevCounters c = b.hctr.read(b.ctr, b.stream);
             std::memset(stats, 0, sizeof(*stats));
             stats->render_ms = ms;
             stats->extension_rays = c.extension_rays;
@@ -833,6 +851,7 @@ struct wf_server {
     int pending = -1;                 // snapshot slot of the last launch, not yet collected
     wf::gpu::DevCounters seen{};      // counters as of the last collected snapshot
     wf::gpu::DevCounters* ctr = nullptr;
+    PinnedCounters hctr;
     SceneCache scenes;
     wf::gpu::PersistentEngine eng;
     std::deque<Req*> queue;
@@ -1012,10 +1031,7 @@ int wf_server_stats_get(wf_server* s, wf_server_stats* out) {
         // While requests run, the counters of the last collected snapshot (reading the device
         // counters would wait for the queued launch); when idle, the device counters.
         wf::gpu::DevCounters c = s->seen;
-        if (s->running.empty()) {
-            wf::gpu::check(hipMemcpyAsync(&c, s->ctr, sizeof(c), hipMemcpyDeviceToHost, s->stream), "copy counters");
-            wf::gpu::check(hipStreamSynchronize(s->stream), "counters sync");
-        }
+        if (s->running.empty()) c = s->hctr.read(s->ctr, s->stream);
         out->extension_rays = c.extension_rays;
         out->shadow_rays = c.shadow_rays;
         out->bytes_in_use = s->in_use;

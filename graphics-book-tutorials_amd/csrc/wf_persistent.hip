@@ -278,6 +278,18 @@ __global__ void pw_init_kernel(PState* st) {
     *st = z;
 }
 
+// Host-side records reach the device as kernel arguments (copied into the launch's argument
+// buffer when the launch is enqueued), never as pageable host-to-device copies: the runtime stages
+// a pageable copy through its own buffers, and nothing in the render path depends on that path.
+template <class T>
+__global__ void pw_put_kernel(T* dst, T v) {
+    static_assert(sizeof(T) % sizeof(int) == 0, "records are whole words");
+    const int* s = reinterpret_cast<const int*>(&v);
+    int* d = reinterpret_cast<int*>(dst);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / sizeof(int)); ++i) d[i] = s[i];
+}
+
 __global__ void pw_admit_kernel(PState* st, JobDev* jobs, uint32_t* job_done, JobDev job, uint32_t index) {
     jobs[index] = job;
     job_done[index] = 0u;
@@ -984,7 +996,8 @@ void PersistentEngine::update_lds_plan() {
         np[k] = m.hscenes[k].n_prims;
     }
     m.lds_plan = plan_lds(nn, np, n, m.lds_bytes);
-    check(hipMemcpyAsync(m.ctx.plan, &m.lds_plan, sizeof(LdsPlan), hipMemcpyHostToDevice, m.stream), "copy lds plan");
+    pw_put_kernel<LdsPlan><<<1, 1, 0, m.stream>>>(m.ctx.plan, m.lds_plan);
+    check(hipGetLastError(), "put lds plan");
 }
 
 // Validates a job, enters its scene in the scene table and returns its device record, its work
@@ -1013,8 +1026,8 @@ JobDev PersistentEngine::prepare_job(const PersistentJob& job, float4* sbuf) {
         sc = (int)m.scene_keys.size();
         m.scene_keys.push_back(job.scene);
         m.hscenes.push_back(*job.scene);
-        check(hipMemcpyAsync(m.ctx.scenes + sc, job.scene, sizeof(SceneView), hipMemcpyHostToDevice, m.stream),
-              "copy scene");
+        pw_put_kernel<SceneView><<<1, 1, 0, m.stream>>>(m.ctx.scenes + sc, *job.scene);
+        check(hipGetLastError(), "put scene record");
         update_lds_plan();
         m.scene_gen += 1;
     }

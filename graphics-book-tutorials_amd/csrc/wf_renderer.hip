@@ -51,6 +51,7 @@ struct GpuRenderer::Impl {
     float* pinned = nullptr;
     hipEvent_t dl_ev[4] = {};   // one per download range (download_to)
     gpu::DevCounters* d_ctr = nullptr;
+    gpu::DevCounters* h_ctr = nullptr;   // pinned host mirror of the counters
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::unique_ptr<gpu::WavefrontEngine> wf;
@@ -65,6 +66,7 @@ struct GpuRenderer::Impl {
         if (film.lum) (void)hipFree(film.lum);
         if (film.lum2) (void)hipFree(film.lum2);
         if (d_ctr) (void)hipFree(d_ctr);
+        if (h_ctr) (void)hipHostFree(h_ctr);
         if (pinned) (void)hipHostFree(pinned);
         for (hipEvent_t e : dl_ev)
             if (e) (void)hipEventDestroy(e);
@@ -93,6 +95,7 @@ GpuRenderer::GpuRenderer(const Scene& scene, GpuMode mode) : impl_(std::make_uni
     gpu::check(hipMalloc(&m.film.lum2, P * sizeof(float)), "hipMalloc film.lum2");
     m.film.pixels = m.pixels;
     gpu::check(hipMalloc(&m.d_ctr, sizeof(gpu::DevCounters)), "hipMalloc counters");
+    gpu::check(hipHostMalloc((void**)&m.h_ctr, sizeof(gpu::DevCounters), hipHostMallocDefault), "hipHostMalloc counters");
     m.own_bytes = m.scene.bytes + 5 * P * sizeof(float) + sizeof(gpu::DevCounters);
     if (mode == GpuMode::Wavefront) {
         m.wf = std::make_unique<gpu::WavefrontEngine>(m.scene.view, m.width, m.height);
@@ -168,8 +171,10 @@ void GpuRenderer::run_passes(int first_sample, int count, const RenderParams& p,
     if (stats) {
         float ms = 0.0f;
         gpu::check(hipEventElapsedTime(&ms, m.ev0, m.ev1), "hipEventElapsedTime");
-        gpu::DevCounters c{};
-        gpu::check(hipMemcpy(&c, m.d_ctr, sizeof(c), hipMemcpyDeviceToHost), "copy counters");
+        gpu::check(hipMemcpyAsync(m.h_ctr, m.d_ctr, sizeof(gpu::DevCounters), hipMemcpyDeviceToHost, m.stream),
+                   "copy counters");
+        gpu::check(hipStreamSynchronize(m.stream), "counters sync");
+        const gpu::DevCounters c = *m.h_ctr;
         stats->render_ms = ms;
         stats->extension_rays += c.extension_rays;
         stats->shadow_rays += c.shadow_rays;
