@@ -659,6 +659,18 @@ int wf_compact_u32(const uint32_t* d_values, const uint32_t* d_flags, uint32_t* 
     return WF_OK;
 }
 
+int wf_compact_u32_noalloc(const uint32_t* d_values, const uint32_t* d_flags, uint32_t* d_out, uint32_t* d_count, int n,
+                           void* stream, int tile) {
+    if (n < 0 || n > wf::gpu::kNoAllocMaxN)
+        return fail(WF_ERR_INVALID, "compact (no allocation): n outside 0..2^26, the size of the library's static partials");
+    const int t = tile > 0 ? tile : wf::gpu::kDefaultScanTile;
+    if (wf::gpu::scan_items_for_tile(t) == 0) return fail(WF_ERR_INVALID, "compact (no allocation): tile must be 512, 1024, 2048 or 4096");
+    hipError_t e = wf::gpu::compact_u32_noalloc(d_values, d_flags, d_out, d_count, n, (hipStream_t)stream, t);
+    if (e == hipErrorInvalidValue) return fail(WF_ERR_INVALID, "compact (no allocation): bad argument");
+    if (e != hipSuccess) return fail(WF_ERR_RUNTIME, hipGetErrorString(e));
+    return WF_OK;
+}
+
 int wf_test_queue_push(const uint32_t* d_flags, int n, uint32_t* d_items, uint32_t capacity, uint32_t* d_counter,
                        uint32_t* d_overflow, void* stream) {
     hipError_t e = wf::gpu::test_queue_push(d_flags, n, d_items, capacity, d_counter, d_overflow, (hipStream_t)stream);

@@ -29,6 +29,14 @@ hipError_t scan_exclusive_u32(const uint32_t* d_in, uint32_t* d_out, int n, uint
 hipError_t compact_u32(const uint32_t* d_values, const uint32_t* d_flags, uint32_t* d_out, uint32_t* d_count, int n,
                        ScanWorkspace& ws, hipStream_t stream, int tile = kDefaultScanTile);
 
+// Allocation-free compaction: as compact_u32, with the per-tile partial sums in a static device
+// array of kNoAllocMaxN / kMinScanTile + 1 words instead of a workspace. n > kNoAllocMaxN ->
+// hipErrorInvalidValue with no side effects. No hipMalloc, hipFree or allocating library call is
+// made; calls that overlap on one device share the array and race.
+constexpr int kNoAllocMaxN = 1 << 26;
+hipError_t compact_u32_noalloc(const uint32_t* d_values, const uint32_t* d_flags, uint32_t* d_out, uint32_t* d_count,
+                               int n, hipStream_t stream, int tile = kDefaultScanTile);
+
 // Test hook: lanes i < n with d_flags[i] != 0 push i through wave_push.
 hipError_t test_queue_push(const uint32_t* d_flags, int n, uint32_t* d_items, uint32_t capacity, uint32_t* d_counter,
                            uint32_t* d_overflow, hipStream_t stream);

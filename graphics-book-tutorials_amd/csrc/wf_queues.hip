@@ -6,9 +6,18 @@
 // Inside a tile every thread owns `items` consecutive elements (one 16-byte load per 4 items),
 // a wave scans the per-thread totals with shuffles, and LDS combines the waves. No block waits
 // on another block, so no launch order or co-residency is assumed.
+#include <mutex>
+
 #include "wf_queues.h"
 
 namespace wf::gpu {
+
+// Partial sums of the allocation-free compaction: one word per tile of the smallest tile size for
+// up to kNoAllocMaxN elements (512 KiB per device, held until the process ends). Each device has
+// its own copy at its own address; host code reaches it only through the address that
+// hipGetSymbolAddress returns on that device (noalloc_partials), never through this name.
+constexpr int kNoAllocPartials = kNoAllocMaxN / kMinScanTile + 1;
+static __device__ __align__(256) uint32_t scratch[kNoAllocPartials];
 
 namespace {
 
@@ -182,13 +191,52 @@ static hipError_t scan_impl(const uint32_t* d_in, uint32_t* d_out, int n, uint32
 
 template <int ITEMS>
 static hipError_t compact_impl(const uint32_t* d_values, const uint32_t* d_flags, uint32_t* d_out, uint32_t* d_count,
-                               int n, ScanWorkspace& ws, hipStream_t stream) {
+                               int n, uint32_t* partial, hipStream_t stream) {
     const int tile = kScanThreads * ITEMS;
     const int nb = (n + tile - 1) / tile;
-    scan_reduce_kernel<ITEMS, true><<<nb, kScanThreads, 0, stream>>>(d_flags, n, ws.partial);
-    scan_partials_kernel<<<1, 1024, 0, stream>>>(ws.partial, nb, d_count);
-    compact_down_kernel<ITEMS><<<nb, kScanThreads, 0, stream>>>(d_values, d_flags, d_out, n, ws.partial);
+    scan_reduce_kernel<ITEMS, true><<<nb, kScanThreads, 0, stream>>>(d_flags, n, partial);
+    scan_partials_kernel<<<1, 1024, 0, stream>>>(partial, nb, d_count);
+    compact_down_kernel<ITEMS><<<nb, kScanThreads, 0, stream>>>(d_values, d_flags, d_out, n, partial);
     return hipGetLastError();
+}
+
+static hipError_t compact_dispatch(const uint32_t* d_values, const uint32_t* d_flags, uint32_t* d_out,
+                                   uint32_t* d_count, int n, uint32_t* partial, hipStream_t stream, int tile) {
+    if (n == 0) {
+        if (d_count) set_u32_kernel<<<1, 1, 0, stream>>>(d_count, 0u);
+        return hipGetLastError();
+    }
+    switch (scan_items_for_tile(tile)) {
+        case 2: return compact_impl<2>(d_values, d_flags, d_out, d_count, n, partial, stream);
+        case 4: return compact_impl<4>(d_values, d_flags, d_out, d_count, n, partial, stream);
+        case 16: return compact_impl<16>(d_values, d_flags, d_out, d_count, n, partial, stream);
+        case 8: return compact_impl<8>(d_values, d_flags, d_out, d_count, n, partial, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// Device address of `scratch` on the current device, looked up once per device (with that device
+// current) and kept; null if the lookup fails.
+static uint32_t* noalloc_partials(hipError_t* err) {
+    static std::mutex mu;
+    static uint32_t* addr[64] = {};
+    static bool looked_up[64] = {};
+    int dev = 0;
+    *err = hipGetDevice(&dev);
+    if (*err != hipSuccess) return nullptr;
+    if (dev < 0 || dev >= 64) {
+        *err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (!looked_up[dev]) {
+        void* p = nullptr;
+        *err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (*err != hipSuccess) return nullptr;
+        addr[dev] = static_cast<uint32_t*>(p);
+        looked_up[dev] = true;
+    }
+    return addr[dev];
 }
 
 hipError_t scan_exclusive_u32(const uint32_t* d_in, uint32_t* d_out, int n, uint32_t* d_total, ScanWorkspace& ws,
@@ -210,17 +258,18 @@ hipError_t scan_exclusive_u32(const uint32_t* d_in, uint32_t* d_out, int n, uint
 hipError_t compact_u32(const uint32_t* d_values, const uint32_t* d_flags, uint32_t* d_out, uint32_t* d_count, int n,
                        ScanWorkspace& ws, hipStream_t stream, int tile) {
     if (n < 0 || n > ws.capacity) return hipErrorInvalidValue;
-    if (n == 0) {
-        if (d_count) set_u32_kernel<<<1, 1, 0, stream>>>(d_count, 0u);
-        return hipGetLastError();
-    }
-    switch (scan_items_for_tile(tile)) {
-        case 2: return compact_impl<2>(d_values, d_flags, d_out, d_count, n, ws, stream);
-        case 4: return compact_impl<4>(d_values, d_flags, d_out, d_count, n, ws, stream);
-        case 16: return compact_impl<16>(d_values, d_flags, d_out, d_count, n, ws, stream);
-        case 8: return compact_impl<8>(d_values, d_flags, d_out, d_count, n, ws, stream);
-        default: return hipErrorInvalidValue;
-    }
+    return compact_dispatch(d_values, d_flags, d_out, d_count, n, ws.partial, stream, tile);
+}
+
+// Every partial that a call reads is written by its own reduce kernel first (tiles [0, nb)), so the
+// array needs no clearing between calls.
+hipError_t compact_u32_noalloc(const uint32_t* d_values, const uint32_t* d_flags, uint32_t* d_out, uint32_t* d_count,
+                               int n, hipStream_t stream, int tile) {
+    if (n < 0 || n > kNoAllocMaxN || scan_items_for_tile(tile) == 0) return hipErrorInvalidValue;
+    hipError_t err = hipSuccess;
+    uint32_t* partial = noalloc_partials(&err);
+    if (!partial) return err != hipSuccess ? err : hipErrorInvalidSymbol;
+    return compact_dispatch(d_values, d_flags, d_out, d_count, n, partial, stream, tile);
 }
 
 hipError_t test_queue_push(const uint32_t* d_flags, int n, uint32_t* d_items, uint32_t capacity, uint32_t* d_counter,

@@ -122,7 +122,8 @@ EXPORTS = [
     "wf_renderer_run_passes", "wf_renderer_download", "wf_renderer_download_mean", "wf_renderer_download_sums",
     "wf_renderer_device_bytes", "wf_renderer_device_name", "wf_renderer_trace", "wf_render_cpu",
     "wf_render_cpu_sums", "wf_finalize", "wf_query_device", "wf_scan_workspace_create",
-    "wf_scan_workspace_destroy", "wf_scan_exclusive_u32", "wf_compact_u32", "wf_test_queue_push",
+    "wf_scan_workspace_destroy", "wf_scan_exclusive_u32", "wf_compact_u32", "wf_compact_u32_noalloc",
+    "wf_test_queue_push",
     "wf_batch_render", "wf_autotune", "wf_server_create", "wf_server_destroy", "wf_server_submit",
     "wf_server_step", "wf_server_poll", "wf_server_stats_get", "wf_test_sincos", "wf_test_sincos_grid_hash",
     "wf_test_intersect", "wf_test_slab", "wf_test_divu32", "wf_test_lds_plan", "wf_test_trace_paths", "wf_copy_probe",
@@ -164,6 +165,7 @@ def lib():
         "wf_scan_workspace_create": (I, [I, C.POINTER(P)]), "wf_scan_workspace_destroy": (None, [P]),
         "wf_scan_exclusive_u32": (I, [P, P, I, P, P, P, I]),
         "wf_compact_u32": (I, [P, P, P, P, P, I, P, P, I]),
+        "wf_compact_u32_noalloc": (I, [P, P, P, P, I, P, I]),
         "wf_test_queue_push": (I, [P, I, P, C.c_uint32, P, P, P]),
         "wf_batch_render": (I, [C.POINTER(Job), I, C.POINTER(RenderParams), C.POINTER(BatchStats)]),
         "wf_autotune": (I, [C.POINTER(Job), I, I, C.POINTER(RenderParams)]),

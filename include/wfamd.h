@@ -185,6 +185,17 @@ int wf_compact_u32(const uint32_t* d_values, const uint32_t* d_flags, uint32_t* 
                    uint32_t* d_scratch, int n, wf_scan_ws* ws, void* stream, int tile);
 int wf_test_queue_push(const uint32_t* d_flags, int n, uint32_t* d_items, uint32_t capacity, uint32_t* d_counter,
                        uint32_t* d_overflow, void* stream);
+/* Stable compaction that allocates nothing: the result of wf_compact_u32, with the per-tile partial
+ * sums in a static array of the library (WF_NOALLOC_MAX_N / 512 + 1 words = 512 KiB on each GPU it
+ * runs on, held until the process ends) instead of a workspace. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc). n above WF_NOALLOC_MAX_N -> WF_ERR_INVALID with no side
+ * effects. The array is shared by every call on one GPU: calls that overlap on one GPU (different
+ * streams or threads) race; calls in one stream order are safe. tile: 0 (default 2048), 512, 1024,
+ * 2048 or 4096. */
+#define WF_NOALLOC_MAX_N (1 << 26)
+int wf_compact_u32_noalloc(const uint32_t* d_values, const uint32_t* d_flags, uint32_t* d_out, uint32_t* d_count, int n,
+                           void* stream, int tile);
 
 /* ---- estimator test hooks (host buffers). device = 0 evaluates on the host, 1 on the GPU.
  *      The reference tests the same functions directly from C++ (tests/test_math_rng.cpp,
