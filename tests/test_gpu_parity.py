@@ -1,17 +1,20 @@
-"""GPU parity: every HIP arm against the CPU arm and the oracle, through the C ABI.
+"""GPU parity: every HIP arm against the oracle (oracle/, a plain-C restatement of the reference),
+through the C ABI.
 
 The contract (bitwise, because every arm runs the same estimator in the same FP32 operation order
 with correctly rounded divide/sqrt and no contraction, and films samples in sample order):
-  megakernel == wavefront == persistent == CPU arm, film sums and ray counts;
+  megakernel == wavefront == persistent == the oracle's render_sums, film sums and ray counts
+  (extension, shadow, invalid), on every scene; the product's CPU arm agrees too;
   wavefront per-bounce integer work (active, queue counts, shadow count, compacted list) == the
-  oracle's CPU wavefront restatement.
+  oracle's CPU wavefront restatement;
+  batch jobs and served requests == the oracle's render of the same view.
 Reference tests followed: tests/test_gpu_render.cu:77-130,215-318, tests/catch/wavefront.cu,
 tests/catch/wavefront_failure.cu.
 """
 import numpy as np
 import pytest
 
-from conftest import assert_bitwise
+from conftest import assert_bitwise, assert_equals_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -87,22 +90,24 @@ def test_intersect_device_equals_host(wf, scenes, name, w, h):
         assert np.array_equal(pd, ph)
 
 
+def counts(st):
+    return (st.extension_rays, st.shadow_rays, st.invalid_hits)
+
+
 @pytest.mark.parametrize("name,w,h", SMALL)
-def test_megakernel_equals_cpu_bitwise(wf, scenes, name, w, h):
+def test_megakernel_equals_oracle_bitwise(wf, scenes, oracle_ref, name, w, h):
     s = scenes(name, w, h)
     spp = 4
-    (grgb, glum, glum2), st = gpu_sums(wf, s, "megakernel", spp)
+    g, st = gpu_sums(wf, s, "megakernel", spp)
+    assert_equals_oracle(g, counts(st), oracle_ref(s, spp), f"{name} megakernel")
     p = wf.params(spp=spp, threads=8)
-    (crgb, clum, clum2), cst = wf.render_cpu_sums(s, 0, spp, p)
-    assert_bitwise(grgb, crgb, f"{name} rgb sums")
-    assert_bitwise(glum, clum, f"{name} lum sums")
-    assert_bitwise(glum2, clum2, f"{name} lum2 sums")
-    assert st.extension_rays == cst.extension_rays and st.shadow_rays == cst.shadow_rays
+    c, cst = wf.render_cpu_sums(s, 0, spp, p)
+    assert_equals_oracle(c, counts(cst), oracle_ref(s, spp), f"{name} CPU arm")
     assert st.invalid_hits == 0
 
 
 @pytest.mark.parametrize("name,w,h", SMALL)
-def test_wavefront_equals_megakernel_and_oracle_trace(wf, po, scenes, name, w, h):
+def test_wavefront_equals_megakernel_and_oracle_trace(wf, po, scenes, oracle_ref, name, w, h):
     s = scenes(name, w, h)
     spp = 2
     mk, mst = gpu_sums(wf, s, "megakernel", spp)
@@ -116,6 +121,8 @@ def test_wavefront_equals_megakernel_and_oracle_trace(wf, po, scenes, name, w, h
     r.close()
     for a, b, k in zip(wfs, mk, ("rgb", "lum", "lum2")):
         assert_bitwise(a, b, f"{name} wavefront {k}")
+    assert_equals_oracle(wfs, (st.extension_rays + st2.extension_rays, st.shadow_rays + st2.shadow_rays,
+                               st.invalid_hits + st2.invalid_hits), oracle_ref(s, spp), f"{name} wavefront")
     assert st.extension_rays + st2.extension_rays == mst.extension_rays
     assert st.shadow_rays + st2.shadow_rays == mst.shadow_rays
     assert st.queue_overflows == 0 and st.invalid_hits == 0
@@ -128,45 +135,47 @@ def test_wavefront_equals_megakernel_and_oracle_trace(wf, po, scenes, name, w, h
 
 @pytest.mark.parametrize("name,w,h", SMALL)
 @pytest.mark.parametrize("pool", [0, 1024])
-def test_persistent_equals_megakernel(wf, scenes, name, w, h, pool):
+def test_persistent_equals_megakernel(wf, scenes, oracle_ref, name, w, h, pool):
     s = scenes(name, w, h)
     spp = 4
     mk, mst = gpu_sums(wf, s, "megakernel", spp)
     pe, st = gpu_sums(wf, s, "persistent", spp, pool_paths=pool)
     for a, b, k in zip(pe, mk, ("rgb", "lum", "lum2")):
         assert_bitwise(a, b, f"{name} persistent pool={pool} {k}")
+    assert_equals_oracle(pe, counts(st), oracle_ref(s, spp), f"{name} persistent pool={pool}")
     assert st.extension_rays == mst.extension_rays and st.shadow_rays == mst.shadow_rays
     assert st.queue_overflows == 0 and st.invalid_hits == 0
 
 
 @pytest.mark.parametrize("depth", [1, 8, 12])
 @pytest.mark.parametrize("graph_iters,pool", [(0, 0), (2, 0), (3, 0), (0, 2048), (6, 2048)])
-def test_persistent_graph_length_and_depth(wf, scenes, depth, graph_iters, pool):
+def test_persistent_graph_length_and_depth(wf, scenes, oracle_ref, depth, graph_iters, pool):
     """Graphs shorter than a render (several launches), the default length (max_depth + 2 when the
-    pool holds every sample, else 16) and depths beyond 8 all give the megakernel's bits."""
+    pool holds every sample, else 16) and depths beyond 8 all give the oracle's bits."""
     s = scenes("divergence", 40, 24)
     mk, mst = gpu_sums(wf, s, "megakernel", 3, depth=depth)
     pe, st = gpu_sums(wf, s, "persistent", 3, depth=depth, graph_iters=graph_iters, pool_paths=pool)
     for a, b, k in zip(pe, mk, ("rgb", "lum", "lum2")):
         assert_bitwise(a, b, f"depth {depth} graph_iters {graph_iters} pool {pool} {k}")
+    assert_equals_oracle(pe, counts(st), oracle_ref(s, 3, depth=depth), f"persistent depth {depth}")
     assert st.extension_rays == mst.extension_rays and st.shadow_rays == mst.shadow_rays
     assert st.queue_overflows == 0 and st.invalid_hits == 0
 
 
 @pytest.mark.parametrize("w,h", [(1, 1), (33, 17), (1, 720), (31, 1)])
-def test_odd_sizes_all_arms_agree(wf, scenes, w, h):
+def test_odd_sizes_all_arms_agree(wf, scenes, oracle_ref, w, h):
     s = scenes("divergence", w, h)
-    ref, _ = gpu_sums(wf, s, "megakernel", 3)
+    ref, mst = gpu_sums(wf, s, "megakernel", 3)
+    assert_equals_oracle(ref, counts(mst), oracle_ref(s, 3), f"megakernel {w}x{h}")
     for mode in ("wavefront", "persistent"):
-        got, _ = gpu_sums(wf, s, mode, 3)
-        for a, b in zip(got, ref):
-            assert_bitwise(a, b, f"{mode} {w}x{h}")
+        got, st = gpu_sums(wf, s, mode, 3)
+        assert_equals_oracle(got, counts(st), oracle_ref(s, 3), f"{mode} {w}x{h}")
     (crgb, _, _), _ = wf.render_cpu_sums(s, 0, 3, wf.params(spp=3, threads=4))
     assert_bitwise(ref[0], crgb, f"cpu {w}x{h}")
 
 
 @pytest.mark.parametrize("mode", ["megakernel", "wavefront", "persistent"])
-def test_progressive_equals_one_shot(wf, scenes, mode):
+def test_progressive_equals_one_shot(wf, po, scenes, oracle_ref, mode):
     s = scenes("sphere_grid", 40, 24)
     r = wf.Renderer(s, mode)
     p = wf.params(spp=6)
@@ -178,9 +187,12 @@ def test_progressive_equals_one_shot(wf, scenes, mode):
     r.close()
     assert_bitwise(rgb, rgb2, "rgb")
     assert_bitwise(var, var2, "var")
+    orgb, ovar = po.finalize(*oracle_ref(s, 6)[0], 6)
+    assert_bitwise(rgb, orgb, "rgb vs oracle")
+    assert_bitwise(var, ovar, "var vs oracle")
 
 
-def test_max_depth_64_and_zero_light_scene(wf):
+def test_max_depth_64_and_zero_light_scene(wf, oracle_ref):
     # Zero-light scene: one Lambert sphere, black environment, no emitters -> every pixel is 0.
     sph = np.zeros((1, 8), np.float32)
     sph[0, :4] = [0, 0, -3, 1]
@@ -195,15 +207,13 @@ def test_max_depth_64_and_zero_light_scene(wf):
         r.close()
         assert not rgb.any() and not var.any(), mode
         assert st.shadow_rays == 0
-    a, _ = gpu_sums(wf, wf.Scene("furnace", 24, 16), "megakernel", 2, depth=64)
-    b, _ = gpu_sums(wf, wf.Scene("furnace", 24, 16), "wavefront", 2, depth=64)
-    c, _ = gpu_sums(wf, wf.Scene("furnace", 24, 16), "persistent", 2, depth=64)
-    for x, y, z in zip(a, b, c):
-        assert_bitwise(x, y, "wavefront depth 64")
-        assert_bitwise(x, z, "persistent depth 64")
+    f = wf.Scene("furnace", 24, 16)
+    for mode in ("megakernel", "wavefront", "persistent"):
+        got, st = gpu_sums(wf, f, mode, 2, depth=64)
+        assert_equals_oracle(got, counts(st), oracle_ref(f, 2, depth=64), f"{mode} depth 64")
 
 
-def test_overflow_is_counted_and_renderer_survives(wf, scenes):
+def test_overflow_is_counted_and_renderer_survives(wf, po, scenes, oracle_ref):
     s = scenes("divergence", 64, 36)
     r = wf.Renderer(s, "wavefront")
     _, _, st = r.render(wf.params(spp=2, queue_capacity=100))
@@ -211,8 +221,7 @@ def test_overflow_is_counted_and_renderer_survives(wf, scenes):
     rgb, var, st2 = r.render(wf.params(spp=2))
     r.close()
     assert st2.queue_overflows == 0
-    ref, _ = gpu_sums(wf, s, "megakernel", 2)
-    assert_bitwise(rgb, wf.finalize(*ref, 2)[0], "after overflow")
+    assert_bitwise(rgb, po.finalize(*oracle_ref(s, 2)[0], 2)[0], "after overflow vs oracle")
 
 
 def test_unknown_mode_and_bad_params_raise(wf, scenes):
@@ -313,8 +322,14 @@ def test_queue_push_count_and_contents(wf, n, cap):
         assert np.array_equal(got, want)
 
 
-# ---- batch and serving: a job gives the same bits alone and in the mix ----
-def test_batch_jobs_equal_standalone_renders(wf, scenes):
+# ---- batch and serving: a job gives the oracle's bits, alone and in the mix ----
+def oracle_image(po, oracle_ref, scene, spp, seed=1234, depth=8):
+    (sums, cnt) = oracle_ref(scene, spp, seed, depth)
+    rgb, var = po.finalize(*sums, spp)
+    return rgb, var, cnt
+
+
+def test_batch_jobs_equal_standalone_renders(wf, po, scenes, oracle_ref):
     specs = [(scenes("cornell555", 40, 30), 3, 1234, 8), (scenes("sphere_grid", 64, 36), 2, 99, 8),
              (scenes("heightfield", 48, 27), 4, 5, 8), (scenes("divergence", 17, 9), 1, 1234, 8),
              (scenes("sphere_grid", 64, 36), 3, 1234, 5)]
@@ -327,11 +342,14 @@ def test_batch_jobs_equal_standalone_renders(wf, scenes):
         r.close()
         assert_bitwise(rgb, want, "batch rgb")
         assert_bitwise(var, wvar, "batch var")
-        total_ext += wst.extension_rays
+        orgb, ovar, ocnt = oracle_image(po, oracle_ref, scene, spp, seed, depth)
+        assert_bitwise(rgb, orgb, "batch rgb vs oracle")
+        assert_bitwise(var, ovar, "batch var vs oracle")
+        total_ext += ocnt[0]
     assert st.extension_rays == total_ext
 
 
-def test_serving_results_equal_alone_and_in_mix(wf, scenes):
+def test_serving_results_equal_alone_and_in_mix(wf, po, scenes, oracle_ref):
     srv = wf.Server(budget_bytes=8 << 30, tuning=wf.params(pool_paths=8192))
     reqs = [(scenes("cornell555", 48, 27), 4, 11), (scenes("sphere_grid", 64, 36), 3, 12),
             (scenes("heightfield", 32, 18), 2, 13), (scenes("sphere_grid", 64, 36), 5, 14),
@@ -354,10 +372,13 @@ def test_serving_results_equal_alone_and_in_mix(wf, scenes):
         r.close()
         assert_bitwise(rgb, want, "served rgb")
         assert_bitwise(var, wvar, "served var")
+        orgb, ovar, _ = oracle_image(po, oracle_ref, s, spp, seed)
+        assert_bitwise(rgb, orgb, "served rgb vs oracle")
+        assert_bitwise(var, ovar, "served var vs oracle")
     srv.close()
 
 
-def test_batch_film_larger_than_one_download_chunk(wf, scenes):
+def test_batch_film_larger_than_one_download_chunk(wf, po, scenes, oracle_ref):
     # A batch job's film is downloaded through the two halves of a fixed 16 MiB pinned buffer in
     # chunks of 419,430 pixels (8 MiB at 20 B per pixel); 1280x720 takes three, so both halves
     # are reused, and 2 spp makes the variance nonzero.
@@ -371,12 +392,15 @@ def test_batch_film_larger_than_one_download_chunk(wf, scenes):
     assert rgb.size == 3 * 1280 * 720 and var.size == 1280 * 720
     assert_bitwise(rgb, want, "chunked rgb")
     assert_bitwise(var, wvar, "chunked var")
+    orgb, ovar, _ = oracle_image(po, oracle_ref, s.view(1280, 720, None), 2, 11)
+    assert_bitwise(rgb, orgb, "chunked rgb vs oracle")
+    assert_bitwise(var, ovar, "chunked var vs oracle")
     assert np.count_nonzero(var) > 0
     assert st.extension_rays == wst.extension_rays and st.shadow_rays == wst.shadow_rays
 
 
 @pytest.mark.parametrize("lds_kib", [0, 24])
-def test_batch_job_views_equal_standalone_view_renders(wf, scenes, lds_kib):
+def test_batch_job_views_equal_standalone_view_renders(wf, po, scenes, oracle_ref, lds_kib):
     # Jobs with their own image size and orbit camera over shared scenes (one BVH per scene): each
     # equals a standalone render of the same view (Scene.view copies geometry, sets the camera).
     # At 24 KiB the LDS plan keeps sphere_grid's whole tree but not its primitive records (the
@@ -397,11 +421,14 @@ def test_batch_job_views_equal_standalone_view_renders(wf, scenes, lds_kib):
         assert rgb.size == 3 * w * h
         assert_bitwise(rgb, want, f"view {n} {w}x{h} yaw {yaw} rgb")
         assert_bitwise(var, wvar, f"view {n} {w}x{h} yaw {yaw} var")
-        total += wst.extension_rays + wst.shadow_rays
+        orgb, ovar, ocnt = oracle_image(po, oracle_ref, v, spp, spec[2])
+        assert_bitwise(rgb, orgb, f"view {n} {w}x{h} yaw {yaw} rgb vs oracle")
+        assert_bitwise(var, ovar, f"view {n} {w}x{h} yaw {yaw} var vs oracle")
+        total += ocnt[0] + ocnt[1]
     assert st.extension_rays + st.shadow_rays == total
 
 
-def test_serving_job_views_equal_standalone(wf, scenes):
+def test_serving_job_views_equal_standalone(wf, po, scenes, oracle_ref):
     base = scenes("sphere_grid", 64, 36)
     srv = wf.Server(budget_bytes=8 << 30, tuning=wf.params(pool_paths=4096))
     reqs = [(48, 27, 10.0, 3), (80, 45, 123.0, 2), (31, 29, None, 4)]
@@ -421,10 +448,13 @@ def test_serving_job_views_equal_standalone(wf, scenes):
         r.close()
         assert_bitwise(rgb, want, "served view rgb")
         assert_bitwise(var, wvar, "served view var")
+        orgb, ovar, _ = oracle_image(po, oracle_ref, base.view(w, h, yaw), spp, 40 + k)
+        assert_bitwise(rgb, orgb, "served view rgb vs oracle")
+        assert_bitwise(var, ovar, "served view var vs oracle")
     srv.close()
 
 
-def test_serving_counts_rays_exactly_and_closes_with_requests_in_flight(wf, scenes):
+def test_serving_counts_rays_exactly_and_closes_with_requests_in_flight(wf, scenes, oracle_ref):
     # A step collects the previous step's launches; once nothing runs, the counters are the
     # device's and equal the standalone renders' ray counts. A server closed mid-flight waits for
     # its streams before freeing.
@@ -442,10 +472,8 @@ def test_serving_counts_rays_exactly_and_closes_with_requests_in_flight(wf, scen
     assert st["in_flight"] == 0 and st["completed"] == len(reqs)
     want = 0
     for spp, seed in reqs:
-        r = wf.Renderer(s, "megakernel")
-        _, _, mst = r.render(wf.params(spp=spp, seed=seed))
-        r.close()
-        want += mst.extension_rays + mst.shadow_rays
+        _, ocnt = oracle_ref(s, spp, seed)
+        want += ocnt[0] + ocnt[1]
     assert st["extension_rays"] + st["shadow_rays"] == want
     srv.close()
     # One step admits and launches; a completion could only be seen by the next step.
@@ -480,14 +508,29 @@ def test_serving_budget_rejects_and_defers(wf, scenes):
     srv.close()
 
 
-# ---- full-size properties on the material-divergent benchmark scene ----
-def test_full_size_sphere_grid_arms_agree(wf, scenes):
+# ---- full sizes: the Cornell box configuration and the material-divergent benchmark scene ----
+def test_cornell_box_configuration_at_full_size_equals_oracle(wf, scenes, oracle_ref):
+    """The Cornell box benchmark configuration exactly: 320x180, 16 spp, depth 8, seed 1234, on
+    every GPU arm and the CPU arm."""
+    print("[cornell555 320x180x16]", flush=True)
+    s = scenes("cornell555", 320, 180)
+    want = oracle_ref(s, 16)
+    for mode in ("megakernel", "wavefront", "persistent"):
+        got, st = gpu_sums(wf, s, mode, 16)
+        assert_equals_oracle(got, counts(st), want, f"cornell555 full size {mode}")
+        assert st.queue_overflows == 0
+    c, cst = wf.render_cpu_sums(s, 0, 16, wf.params(spp=16, threads=16))
+    assert_equals_oracle(c, counts(cst), want, "cornell555 full size CPU arm")
+
+
+def test_full_size_sphere_grid_arms_agree(wf, scenes, oracle_ref):
     s = scenes("sphere_grid", 1280, 720)
     spp = 8
     mk, mst = gpu_sums(wf, s, "megakernel", spp)
     pe, pst = gpu_sums(wf, s, "persistent", spp)
     for a, b in zip(pe, mk):
         assert_bitwise(a, b, "persistent full size")
+    assert_equals_oracle(pe, counts(pst), oracle_ref(s, spp), "sphere_grid 1280x720x8 persistent")
     assert pst.extension_rays == mst.extension_rays and pst.shadow_rays == mst.shadow_rays
     rgb, var = wf.finalize(*mk, spp)
     assert np.isfinite(rgb).all() and np.isfinite(var).all()
