@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <deque>
@@ -163,10 +164,13 @@ struct DevFilmAlloc {
     wf::gpu::DeviceFilm film{};
     int pixels = 0;
     hipStream_t owner = nullptr;   // stream-ordered allocation (serving loop)
-    void alloc(int px, hipStream_t stream) {
+    // pool: the memory pool to allocate from (null: the device's default pool).
+    void alloc(int px, hipStream_t stream, hipMemPool_t pool = nullptr) {
         pixels = px;
         owner = stream;
-        wf::gpu::check(hipMallocAsync((void**)&film.rgb, (size_t)px * 5 * sizeof(float), stream), "hipMallocAsync film");
+        const size_t bytes = (size_t)px * 5 * sizeof(float);
+        if (pool) wf::gpu::check(hipMallocFromPoolAsync((void**)&film.rgb, bytes, pool, stream), "hipMallocFromPoolAsync film");
+        else wf::gpu::check(hipMallocAsync((void**)&film.rgb, bytes, stream), "hipMallocAsync film");
         film.lum = film.rgb + 3 * (size_t)px;
         film.lum2 = film.lum + px;
         film.pixels = px;
@@ -209,6 +213,11 @@ struct DevFilmAlloc {
 };
 
 }  // namespace
+
+uint64_t wf::next_scene_uid() {
+    static std::atomic<uint64_t> next{1};
+    return next.fetch_add(1);
+}
 
 int wf::set_last_error(int code, const std::string& msg) {
     g_err = msg;
@@ -845,17 +854,75 @@ extern "C" int wf_autotune(const wf_job* jobs, int n_jobs, int probe_spp, wf_ren
 // ---------------------------------------------------------------------------------------------
 // serving loop
 // ---------------------------------------------------------------------------------------------
+namespace {
+
+// Device bytes of a scene's upload (DeviceScene::upload: nodes, primitives, materials, lights).
+uint64_t scene_upload_bytes(const wf::Scene& s) {
+    return (uint64_t)s.dnodes.size() * sizeof(wf::Node) + (uint64_t)s.dprims.size() * sizeof(wf::Prim) +
+           (uint64_t)s.materials.size() * sizeof(wf::Material) + (uint64_t)s.lights.size() * sizeof(wf::Light);
+}
+
+// The serving loop's own stream-ordered memory pool: freed request memory stays in it for the next
+// requests (release threshold at the maximum) without changing the device's default pool, which
+// other renderers and batches of the process use. Destroyed after everything allocated from it.
+struct ServerPool {
+    hipMemPool_t pool = nullptr;
+    ServerPool() = default;
+    ServerPool(const ServerPool&) = delete;
+    ServerPool& operator=(const ServerPool&) = delete;
+    void create(int dev) {
+        hipMemPoolProps props{};
+        props.allocType = hipMemAllocationTypePinned;
+        props.handleTypes = hipMemHandleTypeNone;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = dev;
+        wf::gpu::check(hipMemPoolCreate(&pool, &props), "hipMemPoolCreate");
+        uint64_t keep = ~0ull;
+        wf::gpu::check(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep), "hipMemPoolSetAttribute");
+    }
+    ~ServerPool() {
+        if (pool) (void)hipMemPoolDestroy(pool);
+    }
+};
+
+}  // namespace
+
 struct wf_server {
+    // A request renders in consecutive sample ranges (chunks), each an engine job with its own
+    // sample buffer of pixels x chunk spp x 16 B; chunks are resolved into the request's film in
+    // sample order, so the film holds the bits of one render of all its samples, and the memory a
+    // request holds is bounded by its pixels, not its samples.
+    struct Chunk {
+        int64_t seq;      // engine job
+        int spp;
+        uint64_t bytes;
+    };
     struct Req {
         int64_t id;
         wf_job job;
-        int engine_job = -1;
+        uint64_t scene_uid = 0;
+        int px = 0;                      // pixels
+        int chunk_spp = 1;
+        int next_sample = 0;             // samples admitted
+        int resolved = 0;                // samples resolved into the film
+        std::deque<Chunk> chunks;        // admitted, not resolved, in sample order
+        wf::gpu::PersistentJob tmpl;     // the request's view (scene, size, camera, film, seed, depth)
         DevFilmAlloc film;
-        uint64_t bytes = 0;
-        int state = 0;   // 0 queued, 1 admitted, 2 done
+        uint64_t film_bytes = 0;
+        bool film_held = false;          // film allocated and counted in in_use
+        int state = 0;                   // 0 queued, 1 running, 2 done, 3 failed
     };
+    // Uploaded scenes keyed by the handle's uid; `refs` counts the running requests on a scene.
+    // An unused scene stays resident until the budget needs its memory (least recently used first).
+    struct CachedScene {
+        std::unique_ptr<wf::gpu::DeviceScene> dev;
+        int refs = 0;
+        uint64_t last_use = 0;
+    };
+    static constexpr int kChunksInFlight = 2;
     uint64_t budget = 0, in_use = 0;
     wf::gpu::PersistentTuning tune;
+    ServerPool mem;                   // declared before everything allocated from it
     hipStream_t stream = nullptr;
     // Finished requests are resolved and downloaded here while the next launch runs on `stream`.
     hipStream_t side = nullptr;
@@ -864,28 +931,88 @@ struct wf_server {
     wf::gpu::DevCounters seen{};      // counters as of the last collected snapshot
     wf::gpu::DevCounters* ctr = nullptr;
     PinnedCounters hctr;
-    SceneCache scenes;
+    std::map<uint64_t, CachedScene> scenes;
+    uint64_t use_clock = 0;
     wf::gpu::PersistentEngine eng;
     std::deque<Req*> queue;
     std::map<int64_t, std::unique_ptr<Req>> reqs;
-    std::vector<Req*> running;
-    int64_t next_id = 1, submitted = 0, admitted = 0, completed = 0, rejected = 0;
+    std::vector<Req*> running;        // in admission order
+    int64_t next_id = 1, submitted = 0, admitted = 0, completed = 0, rejected = 0, failed = 0;
     double busy_ms = 0.0;
-    int jobs_in_table = 0;
 
     ~wf_server() {
         if (stream) (void)hipStreamSynchronize(stream);
         if (side) (void)hipStreamSynchronize(side);
         for (auto& kv : reqs) kv.second->film.release();
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (side) (void)hipStreamSynchronize(side);
         if (ctr) (void)hipFree(ctr);
         if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
-    uint64_t request_bytes(const wf_job& j) const {
-        const uint64_t px = (uint64_t)(j.width > 0 ? j.width : j.scene->s.width) *
-                            (uint64_t)(j.height > 0 ? j.height : j.scene->s.height);
-        return px * 5 * sizeof(float) + px * (uint64_t)j.spp * sizeof(float4);
+    uint64_t scene_bytes() const {
+        uint64_t b = 0;
+        for (auto& kv : scenes) b += kv.second.dev->bytes;
+        return b;
+    }
+
+    // Frees unused cached scenes, least recently used first, until `need` more bytes fit the
+    // budget; false if they cannot. A scene leaves the engine's scene table before its memory is
+    // freed, after the device has finished every launch that may stage it.
+    bool make_room(uint64_t need) {
+        if (in_use + need <= budget) return true;
+        std::vector<std::pair<uint64_t, uint64_t>> idle;   // (last use, uid)
+        uint64_t freeable = 0;
+        for (auto& kv : scenes)
+            if (kv.second.refs == 0) {
+                idle.push_back({kv.second.last_use, kv.first});
+                freeable += kv.second.dev->bytes;
+            }
+        if (in_use + need > budget + freeable) return false;
+        std::sort(idle.begin(), idle.end());
+        bool synced = false;
+        for (auto& e : idle) {
+            if (in_use + need <= budget) break;
+            CachedScene& c = scenes[e.second];
+            eng.forget_scene(&c.dev->view);
+            if (!synced) {
+                wf::gpu::check(hipStreamSynchronize(stream), "server sync");
+                wf::gpu::check(hipStreamSynchronize(side), "server sync");
+                synced = true;
+            }
+            in_use -= c.dev->bytes;
+            scenes.erase(e.second);
+        }
+        return in_use + need <= budget;
+    }
+
+    Chunk next_chunk(const Req& r) const {
+        Chunk c{};
+        c.spp = std::min(r.chunk_spp, r.job.spp - r.next_sample);
+        c.bytes = (uint64_t)r.px * (uint64_t)c.spp * sizeof(float4);
+        return c;
+    }
+
+    void admit_chunk(Req& r, Chunk c) {
+        wf::gpu::PersistentJob j = r.tmpl;
+        j.first_sample = r.next_sample;
+        j.spp = c.spp;
+        c.seq = eng.admit(j);
+        r.chunks.push_back(c);
+        r.next_sample += c.spp;
+        in_use += c.bytes;
+    }
+
+    // Releases what a request holds; it is reported failed (wf_server_poll < 0).
+    void fail_request(Req* r) {
+        if (r->film_held) {
+            r->film.release();
+            in_use -= r->film_bytes;
+            r->film_held = false;
+        }
+        r->state = 3;
+        failed++;
     }
 };
 
@@ -907,20 +1034,16 @@ int wf_server_create(uint64_t budget_bytes, const wf_render_params* tuning, wf_s
             while (p > (1ull << 20) && p * wf::gpu::PersistentEngine::kBytesPerSlot > budget_bytes / 2) p >>= 1;
             s->tune.pool_paths = (int)p;
         }
-        // Sample buffers and films of requests come from the device's stream-ordered pool; keep
-        // freed memory there for the next requests instead of returning it at every synchronisation.
         int dev = 0;
         wf::gpu::check(hipGetDevice(&dev), "hipGetDevice");
-        hipMemPool_t pool = nullptr;
-        wf::gpu::check(hipDeviceGetDefaultMemPool(&pool, dev), "hipDeviceGetDefaultMemPool");
-        uint64_t keep = ~0ull;
-        wf::gpu::check(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep), "hipMemPoolSetAttribute");
+        s->mem.create(dev);
         wf::gpu::check(hipStreamCreate(&s->stream), "hipStreamCreate");
         wf::gpu::check(hipStreamCreate(&s->side), "hipStreamCreate");
         s->stage.get();   // page-locked once, here, rather than inside the first step
         wf::gpu::check(hipMalloc(&s->ctr, sizeof(wf::gpu::DevCounters)), "hipMalloc counters");
         wf::gpu::check(hipMemsetAsync(s->ctr, 0, sizeof(wf::gpu::DevCounters), s->stream), "hipMemset counters");
         s->eng.open(s->tune, s->ctr, s->stream);
+        s->eng.set_mem_pool(s->mem.pool);
         s->budget = budget_bytes;
         s->in_use = s->eng.pool_bytes();
         if (s->in_use > budget_bytes) throw std::invalid_argument("budget smaller than the path pool");
@@ -938,15 +1061,29 @@ int wf_server_submit(wf_server* s, const wf_job* job, int64_t* id) {
         if (job->width < 0 || job->height < 0 || job->width > 8192 || job->height > 8192)
             throw std::invalid_argument("job image size outside 1..8192");
         s->submitted++;
-        const uint64_t need = s->request_bytes(*job) + s->scenes.bytes();
-        if (s->eng.pool_bytes() + need > s->budget) {
+        const wf::Scene& sc = job->scene->s;
+        const uint64_t px = (uint64_t)(job->width > 0 ? job->width : sc.width) *
+                            (uint64_t)(job->height > 0 ? job->height : sc.height);
+        const uint64_t film = px * 5 * sizeof(float);
+        const uint64_t scene = s->scenes.count(job->scene->uid) ? 0 : scene_upload_bytes(sc);
+        // Samples per chunk: about one pool of paths, fewer if the budget cannot hold one such
+        // chunk beside the film (and the scene, if it is not resident); at least one.
+        const uint64_t pool = s->eng.pool_bytes();
+        const uint64_t free_for_chunk = s->budget > pool + film + scene ? s->budget - pool - film - scene : 0;
+        const uint64_t by_budget = free_for_chunk / (px * sizeof(float4));
+        const uint64_t by_pool = std::max<uint64_t>(1, (uint64_t)s->eng.pool_paths() / px);
+        const uint64_t chunk = std::min<uint64_t>({by_pool, by_budget, (uint64_t)job->spp});
+        if (chunk < 1) {
             s->rejected++;
             return fail(WF_ERR_BUDGET, "request can never fit the device-memory budget");
         }
         auto r = std::make_unique<wf_server::Req>();
         r->id = s->next_id++;
         r->job = *job;
-        r->bytes = s->request_bytes(*job);
+        r->scene_uid = job->scene->uid;
+        r->px = (int)px;
+        r->chunk_spp = (int)chunk;
+        r->film_bytes = film;
         *id = r->id;
         s->queue.push_back(r.get());
         s->reqs.emplace(r->id, std::move(r));
@@ -958,32 +1095,65 @@ int wf_server_step(wf_server* s, int launches) {
     return guard([&] {
         if (!s) throw std::invalid_argument("null server");
         auto t0 = std::chrono::steady_clock::now();
-        // When nothing runs and no launch is outstanding, recycle the job table.
-        if (s->running.empty() && s->pending < 0 && s->jobs_in_table > 0) {
-            s->eng.reset_jobs();
-            s->jobs_in_table = 0;
+        // Next chunks of running requests first (oldest request first, up to kChunksInFlight each),
+        // while they fit the budget and the job ring.
+        bool full = false;
+        for (wf_server::Req* r : s->running) {
+            while (!full && r->next_sample < r->job.spp && (int)r->chunks.size() < wf_server::kChunksInFlight) {
+                const wf_server::Chunk c = s->next_chunk(*r);
+                if (s->in_use + c.bytes > s->budget || !s->eng.can_admit(r->tmpl.scene)) {
+                    full = true;
+                    break;
+                }
+                s->admit_chunk(*r, c);
+            }
         }
-        // Admit queued requests in arrival order while they fit the budget.
-        while (!s->queue.empty() && s->jobs_in_table < 4000) {
+        // Then queued requests in arrival order: the scene (uploaded unless resident), the film and
+        // the first chunk must fit; an unused resident scene may be evicted to make room.
+        while (!full && !s->queue.empty()) {
             wf_server::Req* r = s->queue.front();
-            if (s->in_use + r->bytes > s->budget) break;
+            const auto it = s->scenes.find(r->scene_uid);
+            const bool resident = it != s->scenes.end();
+            const wf_server::Chunk c = s->next_chunk(*r);
+            const uint64_t need = r->film_bytes + c.bytes + (resident ? 0 : scene_upload_bytes(r->job.scene->s));
+            if (!s->make_room(need)) break;
+            if (resident && !s->eng.can_admit(&it->second.dev->view)) break;
+            if (!resident && !s->eng.can_admit(nullptr)) break;   // a new scene needs a free slot
             s->queue.pop_front();
-            const uint64_t scene_before = s->scenes.bytes();
-            const wf::SceneView* v = s->scenes.get(r->job.scene);
-            s->in_use += s->scenes.bytes() - scene_before;   // scene uploads stay resident
-            wf::gpu::PersistentJob j;
-            int w = 0, h = 0;
-            job_view(r->job, &j, &w, &h);
-            r->film.alloc(w * h, s->stream);
-            j.scene = v;
-            j.film = r->film.film;
-            j.spp = r->job.spp;
-            j.seed = r->job.seed;
-            j.max_depth = r->job.max_depth;
-            r->engine_job = s->eng.admit(j);
-            s->jobs_in_table++;
+            bool counted = false;   // the request holds a reference to its cached scene
+            try {
+                wf_server::CachedScene& cs = s->scenes[r->scene_uid];
+                if (!cs.dev) {
+                    auto d = std::make_unique<wf::gpu::DeviceScene>();
+                    d->upload(r->job.scene->s);
+                    s->in_use += d->bytes;
+                    cs.dev = std::move(d);
+                }
+                cs.refs += 1;
+                counted = true;
+                cs.last_use = ++s->use_clock;
+                int w = 0, h = 0;
+                job_view(r->job, &r->tmpl, &w, &h);
+                r->film.alloc(w * h, s->stream, s->mem.pool);
+                r->film_held = true;
+                s->in_use += r->film_bytes;
+                r->tmpl.scene = &cs.dev->view;
+                r->tmpl.film = r->film.film;
+                r->tmpl.seed = r->job.seed;
+                r->tmpl.max_depth = r->job.max_depth;
+                s->admit_chunk(*r, c);
+            } catch (const std::exception&) {
+                // Not admitted after all: give back what it took, report it failed and pass the
+                // error on; the queue and the running requests stay consistent.
+                auto it2 = s->scenes.find(r->scene_uid);
+                if (it2 != s->scenes.end()) {
+                    if (counted) it2->second.refs -= 1;
+                    if (!it2->second.dev) s->scenes.erase(it2);
+                }
+                s->fail_request(r);
+                throw;
+            }
             r->state = 1;
-            s->in_use += r->bytes;
             s->running.push_back(r);
             s->admitted++;
         }
@@ -1000,18 +1170,28 @@ int wf_server_step(wf_server* s, int launches) {
             bool waited = false;
             std::vector<wf_server::Req*> still;
             for (wf_server::Req* r : s->running) {
-                if (s->eng.job_complete(r->engine_job)) {
+                // Resolve complete chunks in sample order; a chunk completed before an older one
+                // waits for it.
+                while (!r->chunks.empty() && s->eng.job_complete(r->chunks.front().seq)) {
                     if (!waited) {   // the side stream starts after the snapshot's launches
                         wf::gpu::check(hipStreamWaitEvent(s->side, s->eng.snapshot_event(slot), 0), "hipStreamWaitEvent");
                         waited = true;
                     }
-                    s->eng.finish_job(r->engine_job, s->side);
+                    const wf_server::Chunk c = r->chunks.front();
+                    s->eng.finish_job(c.seq, s->side);
+                    r->chunks.pop_front();
+                    r->resolved += c.spp;
+                    s->in_use -= c.bytes;
+                }
+                if (r->resolved == r->job.spp) {
                     r->film.download(r->job.spp, r->job.rgb, r->job.lum_var, s->side, s->stage);
                     r->film.owner = s->side;
                     r->film.release();
+                    r->film_held = false;
+                    s->in_use -= r->film_bytes;
                     r->state = 2;
-                    s->in_use -= r->bytes;
                     s->completed++;
+                    s->scenes[r->scene_uid].refs -= 1;
                 } else {
                     still.push_back(r);
                 }
@@ -1028,6 +1208,7 @@ int wf_server_poll(wf_server* s, int64_t id) {
     if (!s) return -1;
     auto it = s->reqs.find(id);
     if (it == s->reqs.end()) return -1;
+    if (it->second->state == 3) return -2;
     return it->second->state == 2 ? 1 : 0;
 }
 

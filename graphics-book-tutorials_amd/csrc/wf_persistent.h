@@ -39,7 +39,12 @@
 namespace wf::gpu {
 
 constexpr int kMaxScenes = 8;
+// Job slots form a ring: a job is admitted with the next sequence number and lives in slot
+// seq & kJobMask until it is finished and every older job has been finished too.
 constexpr int kMaxJobs = 4096;
+constexpr uint32_t kJobMask = kMaxJobs - 1;
+// Deepest path the survivor state can carry (its depth field has 13 bits).
+constexpr int kMaxDepth = 8191;
 
 struct PersistentJob {
     const SceneView* scene = nullptr;   // view with device pointers (one per distinct scene)
@@ -92,11 +97,14 @@ struct JobDev {
 // with seg_prefix[s] <= i < seg_prefix[s + 1] at position seg_base[s] + i - seg_prefix[s].
 constexpr int kSegs = 16;
 
-// Device control block.
+// Device control block. Jobs are numbered by admission (sequence numbers, modulo 2^32): job_hi is
+// the next number, job_lo the oldest job whose paths are not all issued yet, so the new paths of
+// an iteration belong to the jobs [job_lo, job_hi), whose work ranges follow each other.
 struct PState {
     unsigned long long next_work, total_work, it_work_base, pad64;
-    uint32_t n_live, n_jobs, have_prev, iterations;
+    uint32_t n_live, have_prev, iterations, job_lo;
     uint32_t it_live, it_new, it_act, done;
+    uint32_t job_hi, pad1, pad2, pad3;
     uint32_t n_next, n_shadow, shade_block, pad;   // n_next, n_shadow: totals of this iteration (set by shadow)
     uint32_t seg_cnt[kSegs];          // survivors per segment (this iteration)
     uint32_t sh_cnt[kSegs];           // shadow rays per segment (this iteration)
@@ -105,11 +113,21 @@ struct PState {
 };
 
 // One buffer of dense per-item path state (two of them alternate by iteration), 64 bytes a path:
-//   ro : origin xyz, prev_pdf      rd : direction xyz, depth | specular << 16 | job << 17
+//   ro : origin xyz, prev_pdf      rd : direction xyz, depth | specular << 13 | job slot << 14 |
+//                                       scene slot << 26
 //   T  : throughput xyz, key k0    L  : radiance xyz, sample-buffer index
 // k1 of the path's key is rebuilt from the job's seed hash, the pixel and the sample, which the
-// sample-buffer index gives (path_key_k1). Depth <= 0xffff and job < kMaxJobs = 2^12 fit the word.
-static_assert(kMaxJobs <= (1 << 12), "the job index has 12 bits in a survivor's rd.w");
+// sample-buffer index gives (path_key_k1). The scene slot rides along so that a survivor's
+// traversal needs no job lookup.
+constexpr uint32_t kSpecShift = 13, kJobShift = 14, kSceneShift = 26;
+static_assert(kMaxDepth < (1 << kSpecShift), "depth field");
+static_assert(kMaxJobs == (1 << (kSceneShift - kJobShift)), "the job slot has 12 bits in a survivor's rd.w");
+static_assert(kMaxScenes <= 8, "the scene slot has 3 bits in a survivor's rd.w");
+__host__ __device__ inline uint32_t pack_rdw(uint32_t depth, uint32_t specular, uint32_t slot, uint32_t scene) {
+    return depth | (specular << kSpecShift) | (slot << kJobShift) | (scene << kSceneShift);
+}
+// A shadow ray's tag: job slot | scene slot << 12 | final flag (bit 31: the path ended with it).
+constexpr uint32_t kTagSceneShift = 12;
 struct PathBuf {
     float4* ro = nullptr;
     float4* rd = nullptr;
@@ -148,26 +166,34 @@ public:
                 PersistentResult* out, StageTimes* stages = nullptr);
 
     // Session API (continuous batching). open() sizes the pool; admit() appends a job (its
-    // sample buffer is allocated here) and returns its index; run() replays the captured
-    // iteration graph `launches` times; progress() reads the control block and the per-job
-    // finished-path counts; finish_job() resolves a complete job into its film and frees its
-    // sample buffer (on `stream` when given, else the engine's); reset_jobs() clears the job
-    // table once nothing is in flight.
+    // sample buffer is allocated here) and returns its sequence number (the job's id); run()
+    // replays the captured iteration graph `launches` times; progress() reads the control block
+    // and the per-job finished-path counts; finish_job() resolves a complete job into its film and
+    // frees its sample buffer (on `stream` when given, else the engine's); a finished job's slot
+    // is reused once every older job is finished too. can_admit() tells whether a job on `scene`
+    // would find a job slot and a scene slot (a scene slot frees when no unfinished job uses it);
+    // forget_scene() drops a scene no unfinished job uses from the scene table (before its device
+    // memory is freed); reset_jobs() clears the job table once nothing is in flight.
     // For a caller that keeps a launch queued while it works on the host: snapshot() enqueues,
     // after everything enqueued so far, copies of the control block, the per-job finished counts
     // and *ctr (optional) into one of two pinned slots, records an event and returns the slot;
     // collect() waits for that event only (not for later launches) and applies the finished
     // counts of the jobs that existed at the snapshot. At most two snapshots are outstanding.
     void open(const PersistentTuning& t, DevCounters* ctr, hipStream_t stream);
-    int admit(const PersistentJob& job);
+    int64_t admit(const PersistentJob& job);
+    bool can_admit(const SceneView* scene) const;
+    void forget_scene(const SceneView* scene);
+    int64_t live_jobs() const;
+    // Stream-ordered allocations of sample buffers come from `pool` (null: the device's default).
+    void set_mem_pool(hipMemPool_t pool);
     void run(int launches);
     void run_profiled(StageTimes* out);
     PersistentProgress progress();
     int snapshot(const DevCounters* ctr);
     PersistentProgress collect(int slot, DevCounters* ctr_out);
     hipEvent_t snapshot_event(int slot) const;
-    bool job_complete(int job) const;
-    void finish_job(int job, hipStream_t stream = nullptr);
+    bool job_complete(int64_t job) const;
+    void finish_job(int64_t job, hipStream_t stream = nullptr);
     void reset_jobs();
     uint64_t pool_bytes() const { return pool_bytes_; }
     // Device bytes of one path slot: two 64-byte survivor states, the hit and the shadow-queue entry.
@@ -184,7 +210,9 @@ private:
     void ensure_pool(const PersistentTuning& t);
     void capture(bool single);
     void update_lds_plan();
-    JobDev prepare_job(const PersistentJob& job, float4* sbuf);
+    JobDev prepare_job(const PersistentJob& job, float4* sbuf, int64_t* seq);
+    int scene_slot(const SceneView* scene) const;
+    void retire_jobs();
     void admit_chunk(const std::vector<PersistentJob>& chunk);
     uint64_t sbuf_budget_ = 4ull << 30;
     int cap_ = 0;

@@ -58,10 +58,13 @@ __device__ __forceinline__ const SceneView& scene_of(const PCtx& c, int sc) {
     else return c.scenes[sc];
 }
 
-// Single mode reads job 0 (a uniform address), so a captured graph does not depend on the job.
+// Single mode has one unfinished job, the last one admitted: its slot comes from the control block
+// (a uniform address), so a captured graph does not depend on the job.
+__device__ __forceinline__ uint32_t single_slot(const PState* st) { return (st->job_hi - 1u) & kJobMask; }
+
 template <bool SINGLE>
-__device__ __forceinline__ const JobDev& job_of(const PCtx& c, uint32_t jid) {
-    if constexpr (SINGLE) return c.jobs[0];
+__device__ __forceinline__ const JobDev& job_of(const PCtx& c, uint32_t jid, uint32_t slot0) {
+    if constexpr (SINGLE) return c.jobs[slot0];
     else return c.jobs[jid];
 }
 
@@ -77,45 +80,41 @@ __device__ __forceinline__ int find_job(End end, uint32_t n_jobs, unsigned long 
     return lo;
 }
 
-// Multi-job kernels stage the scene table and, for batches of up to kLdsJobs jobs, each job's work
-// end, scene and depth limit in LDS. Otherwise a lane reaches its scene through two dependent
-// memory loads (job record, then scene record), and a new path finds its job by a binary search
-// over the job records in memory: chains of latency before any work starts.
+// Multi-job kernels stage the scene table and, when it has at most kLdsJobs jobs, the work ends of
+// the window of jobs whose paths are not all issued ([job_lo, job_hi)) in LDS: a new path finds its
+// job by a binary search over that window (from memory when it is larger), and a lane reaches its
+// scene record from LDS (survivors carry their scene slot).
 constexpr int kLdsJobs = 256;
 struct JobTab {
     SceneView scenes[kMaxScenes];
-    unsigned long long work_end[kLdsJobs];
-    int scene[kLdsJobs];
-    int max_depth[kLdsJobs];
-    int staged;   // the job arrays hold every job of the batch
+    unsigned long long work_end[kLdsJobs];   // jobs lo, lo + 1, ... of the window
+    uint32_t lo, n;                          // window: sequence number of its first job, its size
+    int staged;                              // work_end holds the whole window
 };
 
-// The whole block calls it. done (optional): the workgroup's finished-path counters, cleared.
-__device__ __forceinline__ void stage_jobs(const PCtx& c, uint32_t n_jobs, JobTab* t, uint32_t* done = nullptr) {
+// The whole block calls it. done, tag (optional): the workgroup's finished-path counters, cleared.
+__device__ __forceinline__ void stage_jobs(const PCtx& c, const PState* st, JobTab* t, uint32_t* done = nullptr,
+                                           int* tag = nullptr) {
     if (done)
-        for (int i = threadIdx.x; i < kLdsJobs; i += blockDim.x) done[i] = 0u;
+        for (int i = threadIdx.x; i < kLdsJobs; i += blockDim.x) {
+            done[i] = 0u;
+            tag[i] = -1;
+        }
     const int n_scenes = c.plan->n;
     const int* src = reinterpret_cast<const int*>(c.scenes);
     int* dst = reinterpret_cast<int*>(t->scenes);
     const int words = n_scenes * (int)(sizeof(SceneView) / sizeof(int));
     for (int i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
-    const bool staged = n_jobs <= (uint32_t)kLdsJobs;
+    const uint32_t lo = st->job_lo, n = st->job_hi - lo;
+    const bool staged = n <= (uint32_t)kLdsJobs;
     if (staged)
-        for (int i = threadIdx.x; i < (int)n_jobs; i += blockDim.x) {
-            t->work_end[i] = c.jobs[i].work_end;
-            t->scene[i] = c.jobs[i].scene;
-            t->max_depth[i] = c.jobs[i].max_depth;
-        }
-    if (threadIdx.x == 0) t->staged = staged ? 1 : 0;
+        for (int i = threadIdx.x; i < (int)n; i += blockDim.x) t->work_end[i] = c.jobs[(lo + i) & kJobMask].work_end;
+    if (threadIdx.x == 0) {
+        t->staged = staged ? 1 : 0;
+        t->lo = lo;
+        t->n = n;
+    }
     __syncthreads();
-}
-
-__device__ __forceinline__ int job_scene(const PCtx& c, const JobTab* t, uint32_t jid) {
-    return t->staged ? t->scene[jid] : c.jobs[jid].scene;
-}
-
-__device__ __forceinline__ int job_depth(const PCtx& c, const JobTab* t, uint32_t jid) {
-    return t->staged ? t->max_depth[jid] : c.jobs[jid].max_depth;
 }
 
 // Pixel of position q in a pass, visiting the image in 8-row bands of 8x8 tiles (edge tiles
@@ -136,8 +135,8 @@ __device__ __forceinline__ void tile_pixel(uint32_t q, uint32_t W, uint32_t H, c
 // Single-job kernels stage the job record in LDS once per workgroup: new paths read its camera,
 // size and divisors at every refill, which would otherwise be a round trip to memory each time.
 // The whole block calls it; the caller synchronises before use.
-__device__ __forceinline__ void stage_job0(const PCtx& c, JobDev* dst) {
-    const int* src = reinterpret_cast<const int*>(c.jobs);
+__device__ __forceinline__ void stage_job0(const PCtx& c, uint32_t slot0, JobDev* dst) {
+    const int* src = reinterpret_cast<const int*>(c.jobs + slot0);
     int* d = reinterpret_cast<int*>(dst);
     for (int i = threadIdx.x; i < (int)(sizeof(JobDev) / sizeof(int)); i += blockDim.x) d[i] = src[i];
 }
@@ -174,25 +173,34 @@ __device__ __forceinline__ JobDev job_fields_scalar(const JobDev* jobs, uint32_t
     d.cam.vertical = Vec3{p->cam.vertical.x, p->cam.vertical.y, p->cam.vertical.z};
     d.first_sample = p->first_sample;
     d.seed = p->seed;
+    d.scene = p->scene;
     return d;
 }
 
-// The path of work index w: its job, its sample-buffer index and its initial state. t: the
-// multi-job kernels' LDS tables; job0: the single-job kernels' LDS copy of job 0. In the multi-job
-// kernels the lanes that start a path together nearly always belong to one job: its record then
-// comes through scalar loads (the constant address space: job records are read-only in these
-// kernels) instead of a round trip to memory in every lane.
+// The path of work index w: its job slot, its sample-buffer index, its scene slot and its initial
+// state. t: the multi-job kernels' LDS tables; job0: the single-job kernels' LDS copy of the job in
+// slot0. In the multi-job kernels the lanes that start a path together nearly always belong to
+// one job: its record then comes through scalar loads (the constant address space: job records are
+// read-only in these kernels) instead of a round trip to memory in every lane.
 template <bool SINGLE>
-__device__ __forceinline__ PathState new_path(const PCtx& c, const JobTab* t, const JobDev* job0, uint32_t n_jobs,
-                                              unsigned long long w, uint32_t* jid_out, uint32_t* sidx_out) {
+__device__ __forceinline__ PathState new_path(const PCtx& c, const JobTab* t, const JobDev* job0, uint32_t slot0,
+                                              unsigned long long w, uint32_t* jid_out, uint32_t* sidx_out,
+                                              uint32_t* scene_out) {
     if constexpr (SINGLE) {
-        return path_of_job(*job0, 0u, w, jid_out, sidx_out);
+        *scene_out = (uint32_t)job0->scene;
+        return path_of_job(*job0, slot0, w, jid_out, sidx_out);
     } else {
-        uint32_t jid;
-        if (t->staged) jid = (uint32_t)find_job([&](int k) { return t->work_end[k]; }, n_jobs, w);
-        else jid = (uint32_t)find_job([&](int k) { return c.jobs[k].work_end; }, n_jobs, w);
+        uint32_t k;
+        if (t->staged) k = (uint32_t)find_job([&](int i) { return t->work_end[i]; }, t->n, w);
+        else k = (uint32_t)find_job([&](int i) { return c.jobs[(t->lo + i) & kJobMask].work_end; }, t->n, w);
+        const uint32_t jid = (t->lo + k) & kJobMask;
         const uint32_t j0 = __builtin_amdgcn_readfirstlane(jid);
-        if (__ballot(jid != j0) == 0) return path_of_job(job_fields_scalar(c.jobs, j0), j0, w, jid_out, sidx_out);
+        if (__ballot(jid != j0) == 0) {
+            const JobDev d = job_fields_scalar(c.jobs, j0);
+            *scene_out = (uint32_t)d.scene;
+            return path_of_job(d, j0, w, jid_out, sidx_out);
+        }
+        *scene_out = (uint32_t)c.jobs[jid].scene;
         return path_of_job(c.jobs[jid], jid, w, jid_out, sidx_out);
     }
 }
@@ -230,26 +238,33 @@ __device__ __forceinline__ LdsNodes src_of(const SceneView& s, const Node* lds, 
                         sp->prim_cnt[sc] > 0 ? lprim + sp->prim_off[sc] : nullptr);
 }
 
-// Adds one per lane with pred to job_done[jid], one atomic per distinct job id in the wave. The
-// multi-job kernels pass their workgroup's LDS counters when the batch's jobs are staged, and add
-// them to the device counters once per job at the end (flush_done): waves of the whole grid would
-// otherwise serialise on the counters of the few jobs in flight.
-__device__ __forceinline__ void count_done(uint32_t* job_done, bool pred, uint32_t jid) {
+// Adds one per lane with pred to the finished-path count of job slot jid, one update per distinct
+// slot in the wave, into the workgroup's LDS counters: a direct-mapped cache of kLdsJobs entries
+// whose tag[e] names the slot entry e counts for (the first slot with e = slot mod kLdsJobs to
+// arrive claims it); a slot whose entry another slot holds adds to its device counter directly.
+// The workgroup adds its counters to the device once at its end (flush_done): waves of the whole
+// grid would otherwise serialise on the device counters of the few jobs in flight.
+__device__ __forceinline__ void count_done(uint32_t* done, int* tag, uint32_t* job_done, bool pred, uint32_t jid) {
     uint64_t m = __ballot(pred);
     while (m) {
         const int leader = __ffsll((unsigned long long)m) - 1;
         const uint32_t j0 = __shfl(jid, leader);
         const uint64_t same = __ballot(pred && jid == j0);
-        if ((int)lane_id() == leader) atomicAdd(&job_done[j0], (uint32_t)__popcll(same));
+        if ((int)lane_id() == leader) {
+            const int e = (int)(j0 & (kLdsJobs - 1));
+            const int owner = atomicCAS(&tag[e], -1, (int)j0);
+            if (owner == -1 || owner == (int)j0) atomicAdd(&done[e], (uint32_t)__popcll(same));
+            else atomicAdd(&job_done[j0], (uint32_t)__popcll(same));
+        }
         m &= ~same;
     }
 }
 
 // Adds the workgroup's LDS counters to the device counters; the whole block calls it at its end.
-__device__ __forceinline__ void flush_done(const uint32_t* done, uint32_t* job_done, uint32_t n_jobs) {
+__device__ __forceinline__ void flush_done(const uint32_t* done, const int* tag, uint32_t* job_done) {
     __syncthreads();
-    for (int i = threadIdx.x; i < (int)n_jobs; i += blockDim.x)
-        if (done[i]) atomicAdd(&job_done[i], done[i]);
+    for (int i = threadIdx.x; i < kLdsJobs; i += blockDim.x)
+        if (tag[i] >= 0 && done[i]) atomicAdd(&job_done[tag[i]], done[i]);
 }
 
 // Output window of each segment for a shade pass over n items in rounds of B: segment s holds the
@@ -290,15 +305,16 @@ __global__ void pw_put_kernel(T* dst, T v) {
     for (int i = 0; i < (int)(sizeof(T) / sizeof(int)); ++i) d[i] = s[i];
 }
 
-__global__ void pw_admit_kernel(PState* st, JobDev* jobs, uint32_t* job_done, JobDev job, uint32_t index) {
-    jobs[index] = job;
-    job_done[index] = 0u;
-    st->n_jobs = index + 1;
+// Job `seq` (its sequence number) into its ring slot.
+__global__ void pw_admit_kernel(PState* st, JobDev* jobs, uint32_t* job_done, JobDev job, uint32_t seq) {
+    jobs[seq & kJobMask] = job;
+    job_done[seq & kJobMask] = 0u;
+    st->job_hi = seq + 1u;
     st->total_work = job.work_end;
     st->done = 0;
 }
 
-__global__ void pw_plan_kernel(PState* st, DevCounters* ctr, uint32_t cap, uint32_t shade_block) {
+__global__ void pw_plan_kernel(PState* st, const JobDev* jobs, DevCounters* ctr, uint32_t cap, uint32_t shade_block) {
     if (st->have_prev) {
         // Fold the previous iteration: its survivors become this iteration's first items, found
         // through the segment windows of its shade pass.
@@ -320,6 +336,8 @@ __global__ void pw_plan_kernel(PState* st, DevCounters* ctr, uint32_t cap, uint3
     }
     st->have_prev = 1;
     st->shade_block = shade_block;
+    // Jobs whose paths have all been issued leave the window that new paths are looked up in.
+    while (st->job_lo != st->job_hi && jobs[st->job_lo & kJobMask].work_end <= st->next_work) st->job_lo += 1u;
     const unsigned long long rem = st->total_work - st->next_work;
     const uint32_t room = cap - st->n_live;
     const uint32_t n_new = (uint32_t)((unsigned long long)room < rem ? (unsigned long long)room : rem);
@@ -347,14 +365,14 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
     __shared__ __align__(16) unsigned char job0_raw[sizeof(JobDev)];   // single-job only
     JobDev& job0 = *reinterpret_cast<JobDev*>(job0_raw);
     const PState* st = c.st;
-    const uint32_t n_act = st->it_act, n_live = st->it_live, n_jobs = st->n_jobs;
+    const uint32_t n_act = st->it_act, n_live = st->it_live, slot0 = single_slot(st);
     const unsigned long long wbase = st->it_work_base;
     if (block_idle(n_act)) return;
     __shared__ uint32_t pre[kSegs + 1], bas[kSegs];
     if (threadIdx.x <= kSegs) pre[threadIdx.x] = st->in_prefix[threadIdx.x];
     if (threadIdx.x < kSegs) bas[threadIdx.x] = st->in_base[threadIdx.x];
-    if constexpr (!SINGLE) stage_jobs(c, n_jobs, &jt);
-    else stage_job0(c, &job0);   // stage_lds synchronises
+    if constexpr (!SINGLE) stage_jobs(c, st, &jt);
+    else stage_job0(c, slot0, &job0);   // stage_lds synchronises
     stage_lds<SINGLE>(c, lds, &sp);   // synchronises
     WaveFeed f;
     feed_first(f, n_act);
@@ -384,11 +402,11 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
                     r.d = Vec3{d.x, d.y, d.z};
                     r.tmin = kRayEps;
                     r.tmax = kInf;
-                    if constexpr (!SINGLE) sc = job_scene(c, &jt, (uint32_t)__float_as_int(d.w) >> 17);
+                    if constexpr (!SINGLE) sc = (int)(((uint32_t)__float_as_int(d.w) >> kSceneShift) & 7u);
                 } else {
-                    uint32_t jid, sidx;
-                    r = new_path<SINGLE>(c, &jt, &job0, n_jobs, wbase + (item - n_live), &jid, &sidx).ray;
-                    if constexpr (!SINGLE) sc = job_scene(c, &jt, jid);
+                    uint32_t jid, sidx, scn;
+                    r = new_path<SINGLE>(c, &jt, &job0, slot0, wbase + (item - n_live), &jid, &sidx, &scn).ray;
+                    if constexpr (!SINGLE) sc = (int)scn;
                 }
                 if constexpr (!SINGLE) {
                     src = src_of(jt.scenes[sc], lds, &sp, sc);
@@ -450,16 +468,17 @@ __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, Path
     __shared__ uint32_t pre[kSegs + 1], bas[kSegs], win[kSegs];
     __shared__ JobTab jt;                   // multi-job only
     __shared__ uint32_t done_cnt[kLdsJobs];   // multi-job only
+    __shared__ int done_tag[kLdsJobs];        // multi-job only
     __shared__ __align__(16) unsigned char job0_raw[sizeof(JobDev)];   // single-job only
     JobDev& job0 = *reinterpret_cast<JobDev*>(job0_raw);
     PState* st = c.st;
-    const uint32_t n = st->it_act, n_live = st->it_live, n_jobs = st->n_jobs;
+    const uint32_t n = st->it_act, n_live = st->it_live, slot0 = single_slot(st);
     const unsigned long long wbase = st->it_work_base;
     const uint32_t wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
     const uint32_t stride = gridDim.x * blockDim.x;
     if (blockIdx.x * blockDim.x >= n) return;   // no round for this workgroup
-    if constexpr (!SINGLE) stage_jobs(c, n_jobs, &jt, done_cnt);
-    else stage_job0(c, &job0);   // synchronised below
+    if constexpr (!SINGLE) stage_jobs(c, st, &jt, done_cnt, done_tag);
+    else stage_job0(c, slot0, &job0);   // synchronised below
     if (threadIdx.x <= kSegs) pre[threadIdx.x] = st->in_prefix[threadIdx.x];
     if (threadIdx.x < kSegs) bas[threadIdx.x] = st->in_base[threadIdx.x];
     if (threadIdx.x == 0) seg_windows(n, blockDim.x, win);
@@ -474,7 +493,7 @@ __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, Path
         shade_load(c, in, pre, bas, item, n, n_live, &cur);
         bool live = false, fin = false;
         int has = 0;
-        uint32_t jid = 0, sidx = 0;
+        uint32_t jid = 0, sidx = 0, sc = 0;
         PathState ps{};
         PathKey key{0u, 0u};
         Ray sh{};
@@ -489,11 +508,12 @@ __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, Path
                 ps.ray.tmax = kInf;
                 ps.prev_pdf = o.w;
                 const uint32_t bits = (uint32_t)__float_as_int(d.w);
-                ps.depth = (int)(bits & 0xffffu);
-                ps.specular = (int)((bits >> 16) & 1u);
+                ps.depth = (int)(bits & ((1u << kSpecShift) - 1u));
+                ps.specular = (int)((bits >> kSpecShift) & 1u);
                 ps.throughput = Vec3{t.x, t.y, t.z};
                 ps.L = Vec3{l.x, l.y, l.z};
-                jid = bits >> 17;
+                jid = (bits >> kJobShift) & kJobMask;
+                sc = (bits >> kSceneShift) & 7u;
                 sidx = (uint32_t)__float_as_int(l.w);
                 // k1 from the job's seed hash, the pixel and the sample (the sample-buffer index
                 // is sample offset * width * height + pixel).
@@ -502,19 +522,14 @@ __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, Path
                 const uint32_t pix = sidx - q * (uint32_t)jb.width * (uint32_t)jb.height;
                 ps.key = PathKey{(uint32_t)__float_as_int(t.w), path_key_k1(jb.key_b, pix, jb.first_sample + q)};
             } else {
-                ps = new_path<SINGLE>(c, &jt, &job0, n_jobs, wbase + (item - n_live), &jid, &sidx);
+                ps = new_path<SINGLE>(c, &jt, &job0, slot0, wbase + (item - n_live), &jid, &sidx, &sc);
             }
             key = ps.key;
             const JobDev& job = SINGLE ? job0 : c.jobs[jid];
             const SceneView* sv;
-            int depth;
-            if constexpr (SINGLE) {
-                sv = &c.scene0;
-                depth = job.max_depth;
-            } else {
-                sv = &jt.scenes[job_scene(c, &jt, jid)];
-                depth = job_depth(c, &jt, jid);
-            }
+            if constexpr (SINGLE) sv = &c.scene0;
+            else sv = &jt.scenes[sc];
+            const int depth = job.max_depth;
             const SceneView& s = *sv;
             const Hit h{hv.x, __float_as_int(hv.y)};
             Surface f;
@@ -566,23 +581,20 @@ __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, Path
             o = bases[round][0] + cn[bin][wave] + mbcnt64(ml);
             out.ro[o] = make_float4(ps.ray.o.x, ps.ray.o.y, ps.ray.o.z, ps.prev_pdf);
             out.rd[o] = make_float4(ps.ray.d.x, ps.ray.d.y, ps.ray.d.z,
-                                    __int_as_float((int)((uint32_t)ps.depth | ((uint32_t)ps.specular << 16) | (jid << 17))));
+                                    __int_as_float((int)pack_rdw((uint32_t)ps.depth, (uint32_t)ps.specular, jid, sc)));
             out.T[o] = make_float4(ps.throughput.x, ps.throughput.y, ps.throughput.z, __int_as_float((int)key.k0));
             out.L[o] = make_float4(ps.L.x, ps.L.y, ps.L.z, __int_as_float((int)sidx));
         }
         if (has) {
             const uint32_t e = bases[round][1] + cn[8][wave] + mbcnt64(ms);
             c.SQ.o[e] = make_float4(sh.o.x, sh.o.y, sh.o.z, sh.tmax);
-            c.SQ.d[e] = make_float4(sh.d.x, sh.d.y, sh.d.z, __int_as_float((int)(jid | (live ? 0u : kFinalFlag))));
+            c.SQ.d[e] = make_float4(sh.d.x, sh.d.y, sh.d.z,
+                                    __int_as_float((int)(jid | (sc << kTagSceneShift) | (live ? 0u : kFinalFlag))));
             c.SQ.c[e] = make_float4(contrib.x, contrib.y, contrib.z, __int_as_float((int)(live ? o : sidx)));
         }
-        if constexpr (!SINGLE) {
-            if (jt.staged) count_done(done_cnt, fin, jid);
-            else count_done(c.job_done, fin, jid);
-        }
+        if constexpr (!SINGLE) count_done(done_cnt, done_tag, c.job_done, fin, jid);
     }
-    if constexpr (!SINGLE)
-        if (jt.staged) flush_done(done_cnt, c.job_done, n_jobs);
+    if constexpr (!SINGLE) flush_done(done_cnt, done_tag, c.job_done);
 }
 
 // Any-hit traversal of the shadow rays, "while-while" like extend. An unoccluded ray adds its
@@ -594,7 +606,9 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
     __shared__ uint32_t pre[kSegs + 1], win[kSegs], shc[kSegs], svc[kSegs];
     __shared__ JobTab jt;                   // multi-job only
     __shared__ uint32_t done_cnt[kLdsJobs];   // multi-job only
+    __shared__ int done_tag[kLdsJobs];        // multi-job only
     PState* st = c.st;
+    const uint32_t slot0 = single_slot(st);
     if (threadIdx.x < kSegs) {
         shc[threadIdx.x] = st->sh_cnt[threadIdx.x];
         svc[threadIdx.x] = st->seg_cnt[threadIdx.x];
@@ -615,13 +629,13 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
             st->n_shadow = acc;
             // A single job: every path that did not survive this iteration is finished once this
             // kernel is done (a pending shadow ray is the last thing it waits for).
-            if constexpr (SINGLE) c.job_done[0] += st->it_act - next;
+            if constexpr (SINGLE) c.job_done[slot0] += st->it_act - next;
         }
     }
     __syncthreads();
     const uint32_t n = pre[kSegs];
     if (block_idle(n)) return;
-    if constexpr (!SINGLE) stage_jobs(c, st->n_jobs, &jt, done_cnt);
+    if constexpr (!SINGLE) stage_jobs(c, st, &jt, done_cnt, done_tag);
     stage_lds<SINGLE>(c, lds, &sp);
     WaveFeed f;
     feed_first(f, n);
@@ -652,7 +666,7 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
                 r.tmin = kRayEps;
                 r.tmax = o.w;
                 if constexpr (!SINGLE) {
-                    const int sc = job_scene(c, &jt, tag & ~kFinalFlag);
+                    const int sc = (int)((tag >> kTagSceneShift) & 7u);
                     src = src_of(jt.scenes[sc], lds, &sp, sc);
                     nn = jt.scenes[sc].n_nodes;
                 }
@@ -670,11 +684,11 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
         uint32_t jid = 0;
         if (item != kIdle && tr.node < 0 && !leaf_parked(leaf)) {
             fin = (tag & kFinalFlag) != 0;
-            if constexpr (!SINGLE) jid = tag & ~kFinalFlag;
+            if constexpr (!SINGLE) jid = tag & kJobMask;
             if (tr.prim < 0) {
                 const float4 cc = c.SQ.c[pos];
                 const uint32_t dest = (uint32_t)__float_as_int(cc.w);
-                float4* dst = fin ? &job_of<SINGLE>(c, jid).sbuf[dest] : &out.L[dest];
+                float4* dst = fin ? &job_of<SINGLE>(c, jid, slot0).sbuf[dest] : &out.L[dest];
                 float4 l = *dst;
                 l.x += cc.x;
                 l.y += cc.y;
@@ -683,13 +697,9 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
             }
             item = kIdle;
         }
-        if constexpr (!SINGLE) {
-            if (jt.staged) count_done(done_cnt, fin, jid);
-            else count_done(c.job_done, fin, jid);
-        }
+        if constexpr (!SINGLE) count_done(done_cnt, done_tag, c.job_done, fin, jid);
     }
-    if constexpr (!SINGLE)
-        if (jt.staged) flush_done(done_cnt, c.job_done, st->n_jobs);
+    if constexpr (!SINGLE) flush_done(done_cnt, done_tag, c.job_done);
 }
 
 // Adds a finished job's samples to its film, per pixel in sample order.
@@ -744,12 +754,19 @@ struct PersistentEngine::Impl {
         JobDev dev;
         uint64_t work = 0;
         uint32_t done = 0;
+        int scene = 0;             // scene slot
         bool finished = false;
         bool owns_buffer = true;   // false: a slice of the render arena
     };
-    std::vector<HostJob> hjobs;
+    // Ring of job slots: job q (sequence number) is ring[q & kJobMask]; the unfinished-or-younger
+    // jobs are [seq_lo, seq_hi).
+    std::vector<HostJob> ring = std::vector<HostJob>(kMaxJobs);
+    int64_t seq_lo = 0, seq_hi = 0;
+    // Scene slots: a slot's key is null once forgotten; refs counts the unfinished jobs on it.
     std::vector<SceneView> hscenes;
     std::vector<const SceneView*> scene_keys;
+    std::vector<int> scene_refs;
+    hipMemPool_t mem_pool = nullptr;   // sample buffers of admitted jobs (null: the default pool)
     unsigned long long total_work = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -757,12 +774,14 @@ struct PersistentEngine::Impl {
     // and the LDS size of every launch), the launch shape and scheduling parameters.
     struct GraphKey {
         bool single = false;
+        int single_scene = -1;
         uint64_t scene_gen = 0;
         int block = 0, refill = 0, leaf_batch = 0, dir_bins = 0, shade_block = 0, cap = 0, iters = 0;
         size_t lds_bytes = 0;
         const void* ctr = nullptr;
         bool operator==(const GraphKey& o) const {
-            return single == o.single && scene_gen == o.scene_gen && block == o.block && refill == o.refill &&
+            return single == o.single && single_scene == o.single_scene && scene_gen == o.scene_gen &&
+                   block == o.block && refill == o.refill &&
                    leaf_batch == o.leaf_batch && dir_bins == o.dir_bins && shade_block == o.shade_block && cap == o.cap && iters == o.iters && lds_bytes == o.lds_bytes &&
                    ctr == o.ctr;
         }
@@ -778,7 +797,7 @@ struct PersistentEngine::Impl {
         uint32_t* done = nullptr;
         DevCounters* ctr = nullptr;
         hipEvent_t ev = nullptr;
-        size_t n_jobs = 0;
+        int64_t seq_hi = 0;     // jobs admitted when the snapshot was taken
         uint64_t jobs_gen = 0;
         bool with_ctr = false, outstanding = false;
     } snap[2];
@@ -802,8 +821,10 @@ struct PersistentEngine::Impl {
     }
     ~Impl() {
         free_graph();
-        for (HostJob& j : hjobs)
+        for (int64_t q = seq_lo; q < seq_hi; ++q) {
+            HostJob& j = ring[q & kJobMask];
             if (j.dev.sbuf && j.owns_buffer) (void)hipFree(j.dev.sbuf);   // the caller's stream may be gone
+        }
         free_arena();
         free_pool();
         if (st_host) (void)hipHostFree(st_host);
@@ -874,9 +895,10 @@ void PersistentEngine::ensure_pool(const PersistentTuning& t) {
     if (cap == cap_ && !m.pool_allocs.empty()) return;
     m.free_graph();
     m.free_pool();
-    // The device scene table lives in the pool allocation.
+    // The device scene table lives in the pool allocation (open() then clears the job table).
     m.scene_keys.clear();
     m.hscenes.clear();
+    m.scene_refs.clear();
     m.scene_gen += 1;
     pool_bytes_ = 0;
     cap_ = cap;
@@ -926,9 +948,13 @@ void PersistentEngine::open(const PersistentTuning& t, DevCounters* ctr, hipStre
 
 void PersistentEngine::reset_jobs() {
     Impl& m = *m_;
-    for (Impl::HostJob& j : m.hjobs)
-        if (j.dev.sbuf && j.owns_buffer) (void)hipFreeAsync(j.dev.sbuf, m.stream);
-    m.hjobs.clear();
+    for (int64_t q = m.seq_lo; q < m.seq_hi; ++q) {
+        Impl::HostJob& j = m.ring[q & kJobMask];
+        if (j.dev.sbuf && j.owns_buffer && !j.finished) (void)hipFreeAsync(j.dev.sbuf, m.stream);
+        j = Impl::HostJob{};
+    }
+    m.seq_lo = m.seq_hi = 0;
+    for (int& r : m.scene_refs) r = 0;
     m.jobs_gen += 1;
     m.sbuf_bytes = 0;
     m.total_work = 0;
@@ -991,7 +1017,7 @@ void PersistentEngine::update_lds_plan() {
     Impl& m = *m_;
     int nn[kMaxScenes] = {}, np[kMaxScenes] = {};
     const int n = (int)m.hscenes.size();
-    for (int k = 0; k < n; ++k) {
+    for (int k = 0; k < n; ++k) {   // a forgotten slot's record is zero: it gets no LDS
         nn[k] = m.hscenes[k].n_nodes;
         np[k] = m.hscenes[k].n_prims;
     }
@@ -1000,32 +1026,73 @@ void PersistentEngine::update_lds_plan() {
     check(hipGetLastError(), "put lds plan");
 }
 
-// Validates a job, enters its scene in the scene table and returns its device record, its work
-// range following the jobs admitted so far; sbuf: its sample buffer, or null to allocate one.
-JobDev PersistentEngine::prepare_job(const PersistentJob& job, float4* sbuf) {
+int PersistentEngine::scene_slot(const SceneView* scene) const {
+    const Impl& m = *m_;
+    if (!scene) return -1;
+    for (size_t k = 0; k < m.scene_keys.size(); ++k)
+        if (m.scene_keys[k] == scene && std::memcmp(&m.hscenes[k], scene, sizeof(SceneView)) == 0) return (int)k;
+    return -1;
+}
+
+// scene null: whether a scene not yet in the table would find a slot.
+bool PersistentEngine::can_admit(const SceneView* scene) const {
+    const Impl& m = *m_;
+    if (m.seq_hi - m.seq_lo >= kMaxJobs) return false;
+    if (scene_slot(scene) >= 0 || (int)m.scene_keys.size() < kMaxScenes) return true;
+    for (int r : m.scene_refs)
+        if (r == 0) return true;
+    return false;
+}
+
+int64_t PersistentEngine::live_jobs() const { return m_->seq_hi - m_->seq_lo; }
+
+void PersistentEngine::set_mem_pool(hipMemPool_t pool) { m_->mem_pool = pool; }
+
+// The record of a forgotten slot is zero (no nodes, no primitives), so no kernel stages or reads
+// the scene's device memory once the table update has run in stream order.
+void PersistentEngine::forget_scene(const SceneView* scene) {
     Impl& m = *m_;
-    if ((int)m.hjobs.size() >= kMaxJobs) throw std::runtime_error("persistent: job table full");
+    const int k = scene_slot(scene);
+    if (k < 0) return;
+    if (m.scene_refs[k] > 0) throw std::logic_error("persistent: forgetting a scene that unfinished jobs use");
+    m.scene_keys[k] = nullptr;
+    m.hscenes[k] = SceneView{};
+    pw_put_kernel<SceneView><<<1, 1, 0, m.stream>>>(m.ctx.scenes + k, m.hscenes[k]);
+    check(hipGetLastError(), "put scene record");
+    update_lds_plan();
+    m.scene_gen += 1;
+}
+
+// Validates a job, enters its scene in the scene table and returns its device record, its work
+// range following the jobs admitted so far, and its sequence number in *seq; sbuf: its sample
+// buffer, or null to allocate one.
+JobDev PersistentEngine::prepare_job(const PersistentJob& job, float4* sbuf, int64_t* seq) {
+    Impl& m = *m_;
+    if (m.seq_hi - m.seq_lo >= kMaxJobs) throw std::runtime_error("persistent: job table full");
     if (job.spp < 1 || job.scene == nullptr) throw std::runtime_error("persistent: invalid job");
-    if (job.max_depth < 0 || job.max_depth > 0xffff) throw std::runtime_error("persistent: max_depth out of range");
+    if (job.max_depth < 0 || job.max_depth > kMaxDepth) throw std::runtime_error("persistent: max_depth out of range");
     const int jw = job.has_view ? job.width : job.scene->width, jh = job.has_view ? job.height : job.scene->height;
     if (jw < 1 || jh < 1) throw std::runtime_error("persistent: job image size must be positive");
     const uint64_t WH = (uint64_t)jw * (uint64_t)jh;
     const uint64_t work = WH * (uint64_t)job.spp;
     if (work >= (1ull << 32)) throw std::runtime_error("persistent: job has 2^32 or more paths");
     // Scene table: kept across job resets; an entry matches by address and by contents (a view
-    // rebuilt at the same address is a different scene).
-    int sc = -1;
-    for (size_t k = 0; k < m.scene_keys.size(); ++k)
-        if (m.scene_keys[k] == job.scene && std::memcmp(&m.hscenes[k], job.scene, sizeof(SceneView)) == 0) sc = (int)k;
+    // rebuilt at the same address is a different scene). A new scene takes a free slot: one past
+    // the end, or one that no unfinished job uses.
+    int sc = scene_slot(job.scene);
     if (sc < 0) {
-        if ((int)m.scene_keys.size() >= kMaxScenes) {
-            if (!m.hjobs.empty()) throw std::runtime_error("persistent: too many distinct scenes");
-            m.scene_keys.clear();
-            m.hscenes.clear();
+        if ((int)m.scene_keys.size() < kMaxScenes) {
+            sc = (int)m.scene_keys.size();
+            m.scene_keys.push_back(nullptr);
+            m.hscenes.push_back(SceneView{});
+            m.scene_refs.push_back(0);
+        } else {
+            for (int k = 0; k < kMaxScenes && sc < 0; ++k)
+                if (m.scene_refs[k] == 0) sc = k;
+            if (sc < 0) throw std::runtime_error("persistent: too many distinct scenes in flight");
         }
-        sc = (int)m.scene_keys.size();
-        m.scene_keys.push_back(job.scene);
-        m.hscenes.push_back(*job.scene);
+        m.scene_keys[sc] = job.scene;
+        m.hscenes[sc] = *job.scene;
         pw_put_kernel<SceneView><<<1, 1, 0, m.stream>>>(m.ctx.scenes + sc, *job.scene);
         check(hipGetLastError(), "put scene record");
         update_lds_plan();
@@ -1053,23 +1120,28 @@ JobDev PersistentEngine::prepare_job(const PersistentJob& job, float4* sbuf) {
     } else {
         // Stream-ordered: no device-wide synchronisation, and finish_job's free returns the memory
         // to the pool for the next job in stream order.
-        check(hipMallocAsync((void**)&d.sbuf, (size_t)work * sizeof(float4), m.stream), "hipMallocAsync sample buffer");
+        const size_t bytes = (size_t)work * sizeof(float4);
+        if (m.mem_pool) check(hipMallocFromPoolAsync((void**)&d.sbuf, bytes, m.mem_pool, m.stream), "hipMallocFromPoolAsync sample buffer");
+        else check(hipMallocAsync((void**)&d.sbuf, bytes, m.stream), "hipMallocAsync sample buffer");
         m.sbuf_bytes += (uint64_t)work * sizeof(float4);
     }
     m.total_work = d.work_end;
+    m.scene_refs[sc] += 1;
     hj.dev = d;
     hj.work = work;
-    m.hjobs.push_back(hj);
+    hj.scene = sc;
+    *seq = m.seq_hi++;
+    m.ring[*seq & kJobMask] = hj;
     return d;
 }
 
-int PersistentEngine::admit(const PersistentJob& job) {
+int64_t PersistentEngine::admit(const PersistentJob& job) {
     Impl& m = *m_;
-    const uint32_t index = (uint32_t)m.hjobs.size();
-    const JobDev d = prepare_job(job, nullptr);
-    pw_admit_kernel<<<1, 1, 0, m.stream>>>(m.st, m.ctx.jobs, m.ctx.job_done, d, index);
+    int64_t seq = 0;
+    const JobDev d = prepare_job(job, nullptr, &seq);
+    pw_admit_kernel<<<1, 1, 0, m.stream>>>(m.st, m.ctx.jobs, m.ctx.job_done, d, (uint32_t)seq);
     check(hipGetLastError(), "persistent admit");
-    return (int)index;
+    return seq;
 }
 
 // Admits a chunk of jobs with their sample buffers in the arena (one allocation, kept across
@@ -1085,22 +1157,30 @@ void PersistentEngine::admit_chunk(const std::vector<PersistentJob>& chunk) {
     }
     uint64_t off = 0;
     for (const PersistentJob& j : chunk) {
-        const uint32_t index = (uint32_t)m.hjobs.size();
-        const JobDev d = prepare_job(j, m.arena + off);
+        int64_t seq = 0;
+        const JobDev d = prepare_job(j, m.arena + off, &seq);
         off += job_pixels(j) * (uint64_t)j.spp;
-        pw_admit_kernel<<<1, 1, 0, m.stream>>>(m.st, m.ctx.jobs, m.ctx.job_done, d, index);
+        pw_admit_kernel<<<1, 1, 0, m.stream>>>(m.st, m.ctx.jobs, m.ctx.job_done, d, (uint32_t)seq);
         check(hipGetLastError(), "persistent admit");
     }
 }
 
+// Single mode (one unfinished job) plans the LDS for that job's scene alone and bakes its record
+// into the launches; multi mode reads the scene table and the plan of every slot from memory.
 PersistentEngine::LaunchPlan PersistentEngine::make_plan(bool single) {
     Impl& m = *m_;
     LaunchPlan L{};
     L.single = single;
     L.c = m.ctx;
     L.c.plan0 = m.lds_plan;
-    if (single) L.c.scene0 = m.hscenes[0];
-    L.lds_bytes = (size_t)m.lds_plan.total_nodes * sizeof(Node) + (size_t)m.lds_plan.total_prims * sizeof(float4);
+    const LdsPlan* plan = &m.lds_plan;
+    if (single) {
+        const SceneView& sv = m.hscenes[m.ring[(m.seq_hi - 1) & kJobMask].scene];
+        L.c.scene0 = sv;
+        L.c.plan0 = plan_lds(&sv.n_nodes, &sv.n_prims, 1, m.lds_bytes);
+        plan = &L.c.plan0;
+    }
+    L.lds_bytes = (size_t)plan->total_nodes * sizeof(Node) + (size_t)plan->total_prims * sizeof(float4);
     const void* fx = single ? (const void*)pw_extend_kernel<true> : (const void*)pw_extend_kernel<false>;
     const void* fs = single ? (const void*)pw_shade_kernel<true> : (const void*)pw_shade_kernel<false>;
     const void* fw = single ? (const void*)pw_shadow_kernel<true> : (const void*)pw_shadow_kernel<false>;
@@ -1124,7 +1204,7 @@ void PersistentEngine::launch_iteration(const LaunchPlan& L, int par, hipEvent_t
     const PathBuf& in = m.S[par & 1];
     const PathBuf& out = m.S[(par & 1) ^ 1];
     if (ev) check(hipEventRecord(ev[0], st), "hipEventRecord");
-    pw_plan_kernel<<<1, 1, 0, st>>>(m.st, m.ctr, c.cap, (uint32_t)m.shade_block);
+    pw_plan_kernel<<<1, 1, 0, st>>>(m.st, c.jobs, m.ctr, c.cap, (uint32_t)m.shade_block);
     if (ev) check(hipEventRecord(ev[1], st), "hipEventRecord");
     if (L.single) pw_extend_kernel<true><<<L.gx, m.block, L.lds_bytes, st>>>(c, in);
     else pw_extend_kernel<false><<<L.gx, m.block, L.lds_bytes, st>>>(c, in);
@@ -1150,9 +1230,10 @@ void PersistentEngine::capture(bool single) {
 
 void PersistentEngine::run(int launches) {
     Impl& m = *m_;
-    if (m.hjobs.empty()) return;
+    if (m.seq_hi == m.seq_lo) return;
     Impl::GraphKey k;
-    k.single = m.hjobs.size() == 1 && m.hscenes.size() == 1;
+    k.single = m.seq_hi - m.seq_lo == 1;
+    k.single_scene = k.single ? m.ring[(m.seq_hi - 1) & kJobMask].scene : -1;
     k.scene_gen = m.scene_gen;
     k.block = m.block;
     k.refill = m.refill;
@@ -1163,6 +1244,7 @@ void PersistentEngine::run(int launches) {
     k.iters = m.graph_iters;
     k.ctr = m.ctx.ctr;
     k.lds_bytes = (size_t)m.lds_plan.total_nodes * sizeof(Node) + (size_t)m.lds_plan.total_prims * sizeof(float4);
+    k.lds_bytes = k.single ? (size_t)m.lds_bytes : k.lds_bytes;   // single mode plans within the budget
     if (!m.exec || !(k == m.graph_key)) {
         capture(k.single);
         m.graph_key = k;
@@ -1174,8 +1256,8 @@ void PersistentEngine::run(int launches) {
 // the admitted work is done. Items per stage come from the control block of each iteration.
 void PersistentEngine::run_profiled(StageTimes* out) {
     Impl& m = *m_;
-    if (m.hjobs.empty()) return;
-    const bool single = m.hjobs.size() == 1 && m.hscenes.size() == 1;
+    if (m.seq_hi == m.seq_lo) return;
+    const bool single = m.seq_hi - m.seq_lo == 1;
     const LaunchPlan L = make_plan(single);
     hipEvent_t ev[5];
     for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
@@ -1224,12 +1306,13 @@ void PersistentEngine::run_profiled(StageTimes* out) {
 
 PersistentProgress PersistentEngine::progress() {
     Impl& m = *m_;
-    const size_t nj = m.hjobs.size();
     check(hipMemcpyAsync(m.st_host, m.st, sizeof(PState), hipMemcpyDeviceToHost, m.stream), "copy progress");
-    if (nj) check(hipMemcpyAsync(m.done_host, m.ctx.job_done, nj * sizeof(uint32_t), hipMemcpyDeviceToHost, m.stream),
-                  "copy job progress");
+    if (m.seq_hi > m.seq_lo)
+        check(hipMemcpyAsync(m.done_host, m.ctx.job_done, kMaxJobs * sizeof(uint32_t), hipMemcpyDeviceToHost, m.stream),
+              "copy job progress");
     check(hipStreamSynchronize(m.stream), "persistent sync");
-    for (size_t k = 0; k < nj; ++k) m.hjobs[k].done = m.done_host[k];
+    for (int64_t q = m.seq_lo; q < m.seq_hi; ++q)
+        if (!m.ring[q & kJobMask].finished) m.ring[q & kJobMask].done = m.done_host[q & kJobMask];
     PersistentProgress p;
     p.next_work = m.st_host->next_work;
     p.total_work = m.st_host->total_work;
@@ -1251,12 +1334,12 @@ int PersistentEngine::snapshot(const DevCounters* ctr) {
         check(hipHostMalloc(&s.ctr, sizeof(DevCounters), hipHostMallocDefault), "hipHostMalloc");
         check(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming), "hipEventCreate");
     }
-    s.n_jobs = m.hjobs.size();
+    s.seq_hi = m.seq_hi;
     s.jobs_gen = m.jobs_gen;
     s.with_ctr = ctr != nullptr;
     check(hipMemcpyAsync(s.st, m.st, sizeof(PState), hipMemcpyDeviceToHost, m.stream), "copy snapshot");
-    if (s.n_jobs)
-        check(hipMemcpyAsync(s.done, m.ctx.job_done, s.n_jobs * sizeof(uint32_t), hipMemcpyDeviceToHost, m.stream),
+    if (m.seq_hi > m.seq_lo)
+        check(hipMemcpyAsync(s.done, m.ctx.job_done, kMaxJobs * sizeof(uint32_t), hipMemcpyDeviceToHost, m.stream),
               "copy snapshot");
     if (ctr) check(hipMemcpyAsync(s.ctr, ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, m.stream), "copy snapshot");
     check(hipEventRecord(s.ev, m.stream), "hipEventRecord snapshot");
@@ -1271,8 +1354,10 @@ PersistentProgress PersistentEngine::collect(int slot, DevCounters* ctr_out) {
     Impl::Snap& s = m.snap[slot];
     check(hipEventSynchronize(s.ev), "snapshot wait");
     s.outstanding = false;
+    // A job admitted after the snapshot is not in it; a slot reused since holds a younger job.
     if (s.jobs_gen == m.jobs_gen)
-        for (size_t k = 0; k < s.n_jobs && k < m.hjobs.size(); ++k) m.hjobs[k].done = s.done[k];
+        for (int64_t q = m.seq_lo; q < s.seq_hi && q < m.seq_hi; ++q)
+            if (!m.ring[q & kJobMask].finished) m.ring[q & kJobMask].done = s.done[q & kJobMask];
     if (ctr_out && s.with_ctr) *ctr_out = *s.ctr;
     PersistentProgress p;
     p.next_work = s.st->next_work;
@@ -1285,14 +1370,30 @@ PersistentProgress PersistentEngine::collect(int slot, DevCounters* ctr_out) {
 
 hipEvent_t PersistentEngine::snapshot_event(int slot) const { return m_->snap[slot & 1].ev; }
 
-bool PersistentEngine::job_complete(int job) const {
-    const Impl::HostJob& j = m_->hjobs[job];
-    return (uint64_t)j.done == j.work;
+bool PersistentEngine::job_complete(int64_t job) const {
+    const Impl& m = *m_;
+    if (job < m.seq_lo) return true;   // finished and retired
+    if (job >= m.seq_hi) throw std::logic_error("persistent: no such job");
+    const Impl::HostJob& j = m.ring[job & kJobMask];
+    return j.finished || (uint64_t)j.done == j.work;
 }
 
-void PersistentEngine::finish_job(int job, hipStream_t stream) {
+// Jobs finish in any order; their slots return to the ring oldest first.
+void PersistentEngine::retire_jobs() {
     Impl& m = *m_;
-    Impl::HostJob& j = m.hjobs[job];
+    while (m.seq_lo < m.seq_hi && m.ring[m.seq_lo & kJobMask].finished) {
+        Impl::HostJob& j = m.ring[m.seq_lo & kJobMask];
+        m.scene_refs[j.scene] -= 1;
+        j = Impl::HostJob{};
+        m.seq_lo += 1;
+    }
+}
+
+void PersistentEngine::finish_job(int64_t job, hipStream_t stream) {
+    Impl& m = *m_;
+    if (job < m.seq_lo) return;
+    if (job >= m.seq_hi) throw std::logic_error("persistent: no such job");
+    Impl::HostJob& j = m.ring[job & kJobMask];
     if (j.finished) return;
     hipStream_t on = stream ? stream : m.stream;
     const int WH = j.dev.width * j.dev.height;
@@ -1304,6 +1405,7 @@ void PersistentEngine::finish_job(int job, hipStream_t stream) {
     }
     j.dev.sbuf = nullptr;
     j.finished = true;
+    retire_jobs();
 }
 
 void PersistentEngine::render(const PersistentJob* jobs, int n_jobs, const PersistentTuning& t, DevCounters* ctr,
@@ -1367,10 +1469,10 @@ void PersistentEngine::render(const PersistentJob* jobs, int n_jobs, const Persi
             }
         }
         res.iterations += (int)pr.iterations;
-        for (int k = 0; k < (int)m_->hjobs.size(); ++k) {
-            if (!job_complete(k)) throw std::runtime_error("persistent: job incomplete after the work drained");
-            finish_job(k);
-        }
+        const int64_t lo = m_->seq_lo, hi = m_->seq_hi;
+        for (int64_t q = lo; q < hi; ++q)
+            if (!job_complete(q)) throw std::runtime_error("persistent: job incomplete after the work drained");
+        for (int64_t q = lo; q < hi; ++q) finish_job(q);
         res.chunks += 1;
     }
     reset_jobs();

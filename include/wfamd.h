@@ -274,7 +274,13 @@ int wf_batch_render(const wf_job* jobs, int n_jobs, const wf_render_params* tuni
  * them is written into *tuning (pool_paths 0: sized to the work). Renders into the jobs' films. */
 int wf_autotune(const wf_job* jobs, int n_jobs, int probe_spp, wf_render_params* tuning);
 
-/* ---- serving loop: continuous batching under a device-memory budget (no reference counterpart) ---- */
+/* ---- serving loop: continuous batching under a device-memory budget (no reference counterpart) ----
+ * A request renders in consecutive sample ranges of about one path pool each, resolved into its
+ * film in sample order (the same bits as one render of all its samples): the device memory it
+ * holds is its film (20 B per pixel) plus at most two ranges' sample buffers (16 B per pixel and
+ * sample of the range), whatever its spp. The server keeps requests' scenes uploaded (keyed by the
+ * scene handle, not its address) while the budget allows. A scene handle must outlive every
+ * request that names it (until wf_server_poll reports it done or failed). */
 typedef struct wf_server_stats {
     int64_t submitted, admitted, completed, rejected;
     int64_t in_flight;
@@ -285,14 +291,17 @@ typedef struct wf_server_stats {
 
 int wf_server_create(uint64_t budget_bytes, const wf_render_params* tuning, wf_server** out);
 void wf_server_destroy(wf_server* s);
-/* Queues a request; returns its id in *id. WF_ERR_BUDGET if it can never fit the budget. */
+/* Queues a request; returns its id in *id. WF_ERR_BUDGET if it can never fit the budget (its film,
+ * one sample of sample buffer and its scene beside the path pool). */
 int wf_server_submit(wf_server* s, const wf_job* job, int64_t* id);
 /* Admits queued requests that fit, queues `launches` graph replays, and then finishes the requests
  * that the previous step's replays completed (resolves and downloads their results on a second
  * stream while the replays just queued run). A request is therefore reported done by the step
  * after the one whose replays completed it; step(s, 0) only collects. */
 int wf_server_step(wf_server* s, int launches);
-/* 1 if request `id` is done (its buffers are filled), 0 if not yet, negative if unknown. */
+/* 1 if request `id` is done (its buffers are filled), 0 if not yet, -2 if it failed (a device error
+ * while admitting it, which that wf_server_step call returned; what it held is released), -1 if
+ * the id is unknown. */
 int wf_server_poll(wf_server* s, int64_t id);
 /* Ray counts are those of the last collected step while requests run, and exact when none do. */
 int wf_server_stats_get(wf_server* s, wf_server_stats* out);

@@ -57,3 +57,68 @@ def test_batch_configuration_64_jobs_one_autotuned_launch_equal_oracle(wf, po, o
         sh += ocnt[1]
     print(f"[batch] oracle: {ext + sh} rays in {time.perf_counter() - t0:.1f} s", flush=True)
     assert (st.extension_rays, st.shadow_rays) == (ext, sh)
+
+
+def test_serving_configuration_poisson_arrivals_under_a_binding_budget(wf, po, oracle_ref):
+    """A dozen requests of the serving configuration (seed 4242) arriving as a Poisson process
+    at 100 requests/s under a 768 MiB budget (a path pool of 2^21 slots and 400 MB for films and
+    sample buffers), which holds a few requests at a time: the others wait in the queue."""
+    import wfbench
+    cfg = dict(wfbench.PROTOCOL["configs"]["4"])
+    rng = np.random.default_rng(cfg["request_seed"])
+    n = 12
+    views = wfbench.make_views(rng, n, cfg, False)
+    base = {s: wf.Scene(s, 64, 36) for s in cfg["scenes"]}
+    seed0 = wfbench.PROTOCOL["seed"]
+    paths = [w * h * spp for _, w, h, _, spp in views]
+    print(f"[serving] {n} requests, {sum(paths)} paths, largest {max(paths)}", flush=True)
+    budget = 768 << 20
+    # Arrivals at 100 requests/s: Poisson, seeded.
+    arrivals = np.cumsum(rng.exponential(1.0 / 100.0, size=n))
+    srv = wf.Server(budget_bytes=budget)
+    ids, t_done = [None] * n, [None] * n
+    nxt, peak, peak_bytes, waited = 0, 0, 0, 0
+    t0 = time.perf_counter()
+    while True:
+        now = time.perf_counter() - t0
+        while nxt < n and arrivals[nxt] <= now:
+            s, w, h, yaw, spp = views[nxt]
+            ids[nxt] = srv.submit(base[s], spp, seed0 + nxt, 8, w, h, yaw)
+            nxt += 1
+        srv.step(1)
+        now = time.perf_counter() - t0
+        st = srv.stats()
+        peak, peak_bytes = max(peak, st["in_flight"]), max(peak_bytes, st["bytes_in_use"])
+        waited += 1 if st["submitted"] - st["admitted"] > 0 else 0
+        for k in range(nxt):
+            if t_done[k] is None:
+                p = srv.poll(ids[k])
+                assert p >= 0, f"request {k} failed"
+                if p == 1:
+                    t_done[k] = now
+        if nxt == n and all(t is not None for t in t_done):
+            break
+        if nxt < n and st["in_flight"] == 0:
+            time.sleep(max(0.0, min(arrivals[nxt] - (time.perf_counter() - t0), 0.002)))
+        assert now < 300, "serving did not finish"
+    lat = [(t_done[k] - arrivals[k]) * 1e3 for k in range(n)]
+    print(f"[serving] p50 {np.percentile(lat, 50):.1f} ms p99 {np.percentile(lat, 99):.1f} ms, "
+          f"up to {peak} on the GPU, peak {peak_bytes / 2**30:.2f} GiB, {waited} steps with requests queued",
+          flush=True)
+    assert peak_bytes <= budget and peak < n and waited > 0
+    # Every request equals its standalone render; the four smallest equal the oracle.
+    smallest = set(sorted(range(n), key=lambda k: paths[k])[:4])
+    for k in range(n):
+        s, w, h, yaw, spp = views[k]
+        rgb, var = srv.result(ids[k])
+        v = base[s].view(w, h, yaw)
+        r = wf.Renderer(v, "megakernel")
+        want, wvar, _ = r.render(wf.params(spp=spp, seed=seed0 + k, max_depth=8))
+        r.close()
+        assert_bitwise(rgb, want, f"request {k} {s} {w}x{h} spp {spp} alone vs in the mix")
+        assert_bitwise(var, wvar, f"request {k} var alone vs in the mix")
+        if k in smallest:
+            orgb, ovar, _ = oracle_image(po, oracle_ref, v, spp, seed0 + k)
+            assert_bitwise(rgb, orgb, f"request {k} rgb vs oracle")
+            assert_bitwise(var, ovar, f"request {k} var vs oracle")
+    srv.close()

@@ -487,12 +487,23 @@ def test_serving_counts_rays_exactly_and_closes_with_requests_in_flight(wf, scen
 
 def test_serving_budget_rejects_and_defers(wf, scenes):
     s = scenes("sphere_grid", 64, 36)
+    # One request's footprint, measured: its film and the sample buffers of its chunks in flight
+    # (the pool and the resident scene stay when it is done).
     probe = wf.Server(budget_bytes=1 << 40, tuning=wf.params(pool_paths=4096))
-    pool = probe.stats()["bytes_in_use"]
+    pid = probe.submit(s, 8, 0)
+    peak1 = 0
+    for _ in range(5000):
+        probe.step(1)
+        peak1 = max(peak1, probe.stats()["bytes_in_use"])
+        if probe.poll(pid) == 1:
+            break
+    idle = probe.stats()["bytes_in_use"]
     probe.close()
-    req = 64 * 36 * 5 * 4 + 64 * 36 * 8 * 16          # film sums + sample buffer of one request
-    scene_bytes = 200_000                              # sphere_grid upload is ~115 KB
-    srv = wf.Server(budget_bytes=pool + scene_bytes + int(1.5 * req), tuning=wf.params(pool_paths=4096))
+    # A request is admitted with its film and its first chunk (one sample here: the pool holds 4096
+    # paths, fewer than two samples of 64x36), and takes its second chunk from what is left.
+    first = 64 * 36 * 5 * 4 + 64 * 36 * 16
+    assert peak1 - idle == first + 64 * 36 * 16
+    srv = wf.Server(budget_bytes=idle + first + first // 2, tuning=wf.params(pool_paths=4096))
     with pytest.raises(wf.WfError) as e:
         srv.submit(wf.Scene("sphere_grid", 1920, 1080), 256)
     assert e.value.code == wf.WF_ERR_BUDGET
