@@ -96,6 +96,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-spp", type=int, default=16, help="samples per pixel of the CPU baseline sample")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--pool-paths", type=int, default=0, help="persistent path slots (0: sized to the work)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run the plain-copy calibration kernel over BYTES (traffic-counter scale)")
     args = ap.parse_args()
@@ -117,12 +118,13 @@ def main():
     scene = wfamd.Scene(args.scene, args.width, args.height)
     spp, depth, seed = args.spp, 8, 1234
 
-    tuning = wfamd.params(spp=spp, seed=seed, max_depth=depth)
+    tuning = wfamd.params(spp=spp, seed=seed, max_depth=depth, pool_paths=args.pool_paths)
     if not args.no_autotune:
         t0 = time.perf_counter()
         tuned = wfamd.autotune([(scene, spp, seed, depth)], probe_spp=16)
         for k in TUNING_KEYS:
-            setattr(tuning, k, getattr(tuned, k))
+            if k != "pool_paths" or not args.pool_paths:
+                setattr(tuning, k, getattr(tuned, k))
         print(f"[bench] autotune: {', '.join(f'{k} {getattr(tuned, k)}' for k in TUNING_KEYS)} "
               f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
 

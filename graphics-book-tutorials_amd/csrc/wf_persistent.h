@@ -104,7 +104,7 @@ struct PState {
     unsigned long long next_work, total_work, it_work_base, pad64;
     uint32_t n_live, have_prev, iterations, job_lo;
     uint32_t it_live, it_new, it_act, done;
-    uint32_t job_hi, pad1, pad2, pad3;
+    uint32_t job_hi, unocc, pad2, pad3;   // unocc: unoccluded shadow rays of this iteration (set by shadow)
     uint32_t n_next, n_shadow, shade_block, pad;   // n_next, n_shadow: totals of this iteration (set by shadow)
     uint32_t seg_cnt[kSegs];          // survivors per segment (this iteration)
     uint32_t sh_cnt[kSegs];           // shadow rays per segment (this iteration)

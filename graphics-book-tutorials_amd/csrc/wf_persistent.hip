@@ -348,6 +348,7 @@ __global__ void pw_plan_kernel(PState* st, const JobDev* jobs, DevCounters* ctr,
     st->next_work += n_new;
     st->n_next = 0;
     st->n_shadow = 0;
+    st->unocc = 0;
     for (int s = 0; s < kSegs; ++s) {
         st->seg_cnt[s] = 0;
         st->sh_cnt[s] = 0;
@@ -607,6 +608,7 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
     __shared__ JobTab jt;                   // multi-job only
     __shared__ uint32_t done_cnt[kLdsJobs];   // multi-job only
     __shared__ int done_tag[kLdsJobs];        // multi-job only
+    __shared__ uint32_t blk_unocc;
     PState* st = c.st;
     const uint32_t slot0 = single_slot(st);
     if (threadIdx.x < kSegs) {
@@ -646,6 +648,8 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
         nn = c.scene0.n_nodes;
     }
     uint32_t item = kIdle, pos = 0, tag = 0;
+    uint32_t unocc = 0;   // unoccluded rays of this wave (scalar), for the byte count of a profiled iteration
+    if (threadIdx.x == 0) blk_unocc = 0u;
     Traversal tr;
     tr.node = -1;
     tr.prim = -1;
@@ -680,12 +684,13 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
             continue;   // slice used up: take the next one
         }
         wave_traverse(nn, src, tr, leaf, true, (f.dry && f.cursor >= f.end) ? 1 : c.refill, c.leaf_batch);
-        bool fin = false;
+        bool fin = false, clear = false;
         uint32_t jid = 0;
         if (item != kIdle && tr.node < 0 && !leaf_parked(leaf)) {
             fin = (tag & kFinalFlag) != 0;
             if constexpr (!SINGLE) jid = tag & kJobMask;
             if (tr.prim < 0) {
+                clear = true;
                 const float4 cc = c.SQ.c[pos];
                 const uint32_t dest = (uint32_t)__float_as_int(cc.w);
                 float4* dst = fin ? &job_of<SINGLE>(c, jid, slot0).sbuf[dest] : &out.L[dest];
@@ -697,8 +702,12 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
             }
             item = kIdle;
         }
+        unocc += (uint32_t)__popcll(__ballot(clear));
         if constexpr (!SINGLE) count_done(done_cnt, done_tag, c.job_done, fin, jid);
     }
+    if (lane_id() == 0 && unocc) atomicAdd(&blk_unocc, unocc);
+    __syncthreads();
+    if (threadIdx.x == 0 && blk_unocc) atomicAdd(&st->unocc, blk_unocc);
     if constexpr (!SINGLE) flush_done(done_cnt, done_tag, c.job_done);
 }
 
@@ -1280,12 +1289,12 @@ void PersistentEngine::run_profiled(StageTimes* out) {
         // Algorithmic bytes (PathBuf and ShadowQueue layouts; BVH and primitives come from LDS):
         //   extend  survivor ray 32 (new paths are generated), hit 8
         //   shade   hit 8; survivor state 64 in, 64 out; sample 16 per finished path; shadow ray 48
-        //   shadow  ray 32 (origin, direction); the contribution and the radiance read-modify-write
-        //           of unoccluded rays are not counted
-        const long long live = s.it_live, act = s.it_act, next = s.n_next, sh = s.n_shadow;
+        //   shadow  ray 32 (origin, direction) and, for an unoccluded ray, its contribution 16 and
+        //           the radiance read-modify-write 32
+        const long long live = s.it_live, act = s.it_act, next = s.n_next, sh = s.n_shadow, un = s.unocc;
         out->bytes[0] += 32 * live + 8 * act;
         out->bytes[2] += 8 * act + 64 * live + 64 * next + 16 * (act - next) + 48 * sh;
-        out->bytes[5] += 32 * sh;
+        out->bytes[5] += 32 * sh + 48 * un;
         out->items[7] += 1;
         out->launches[0] += 1;
         out->launches[2] += 1;
