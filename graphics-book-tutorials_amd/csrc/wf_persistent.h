@@ -12,10 +12,11 @@
 //           survivor's radiance (or to the sample of a path that ended with it)
 // Survivor and shadow-ray slots are reserved once per workgroup round (the waves' ballot counts are
 // combined in LDS, one atomic per counter per round), so every state read and write of extend and
-// shade is a contiguous run. Traversal kernels run "while-while" with dynamic fetch: every wave
-// owns a contiguous range of rays and a lane whose ray has finished takes the next one as soon as
-// fewer than `refill` lanes of the wave are still traversing, so the wave does not idle on its
-// longest ray. Inside the loop a lane that reaches a hit leaf parks it, and the wave tests parked
+// shade is a contiguous run. Traversal kernels run "while-while" with dynamic fetch: a workgroup
+// owns every B-th slice of 128 rays and its waves take those slices in turn from an LDS counter; a
+// lane whose ray has finished takes the next ray of its wave's slice as soon as fewer than `refill`
+// lanes of the wave are still traversing, so neither a wave nor a workgroup idles on its longest
+// ray. Inside the loop a lane that reaches a hit leaf parks it, and the wave tests parked
 // leaves together once `leaf_batch` lanes have one; each ray still visits the nodes and tests the
 // primitives of the shared traversal in the same order, so hits are the megakernel's bits.
 // Every stage kernel reads its item count from the device control block, so an iteration needs

@@ -9,7 +9,6 @@
 // before the next classify, so every path computes the megakernel's bits.
 #include <algorithm>
 #include <cstring>
-#include <type_traits>
 
 #include "wf_persistent.h"
 
@@ -289,13 +288,12 @@ __device__ __forceinline__ uint32_t seg_pos(const uint32_t* prefix, const uint32
     return base[s] + (i - prefix[s]);
 }
 
-// Slices of kFeedSlice items are dealt to the waves of a single-job traversal kernel per
-// workgroup: workgroup b owns the slices b, b + B, b + 2B, ... (B workgroups), which sample the whole
-// item range, and its waves take them in turn from a counter in LDS, the first one by wave index.
+// Slices of kFeedSlice items are dealt to the waves of a traversal kernel per workgroup: workgroup
+// b owns the slices b, b + B, b + 2B, ... (B workgroups), which sample the whole item range, and its
+// waves take them in turn from a counter in LDS, the first one by wave index.
 // A wave that drew cheap rays so takes more of its workgroup's slices, and the workgroup ends when
 // its work ends rather than when its unluckiest wave finishes a fixed share. Every item is traced
-// once, by whichever wave. Wave-uniform state; the whole wave calls these. (Multi-job kernels keep
-// the static round-robin WaveFeed: the counter's registers would cost them a wave per SIMD.)
+// once, by whichever wave. Wave-uniform state; the whole wave calls these.
 struct BlockFeed {
     uint32_t cursor = 0, end = 0;   // rest of the current slice [cursor, end)
     bool dry = false;               // no slice left
@@ -323,17 +321,9 @@ __device__ __forceinline__ void feed_next(BlockFeed& f, uint32_t n, uint32_t* ct
     feed_take(f, n, (uint32_t)__builtin_amdgcn_readfirstlane((int)j));
 }
 
-__device__ __forceinline__ void feed_next(WaveFeed& f, uint32_t n, uint32_t*) { feed_next(f, n); }
-
-template <bool SINGLE>
-using TraversalFeed = std::conditional_t<SINGLE, BlockFeed, WaveFeed>;
-
-// True when no wave of this workgroup has a slice of the feed (it returns before staging LDS):
-// under BlockFeed the workgroup's first slice is blockIdx.x, under WaveFeed its first wave's.
-template <bool SINGLE>
+// True when the workgroup owns no slice (its first is blockIdx.x): it returns before staging LDS.
 __device__ __forceinline__ bool feed_idle(uint32_t n) {
-    if constexpr (SINGLE) return (unsigned long long)blockIdx.x * kFeedSlice >= n;
-    else return block_idle(n);
+    return (unsigned long long)blockIdx.x * kFeedSlice >= n;
 }
 
 __global__ void pw_init_kernel(PState* st) {
@@ -416,15 +406,15 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
     const PState* st = c.st;
     const uint32_t n_act = st->it_act, n_live = st->it_live, slot0 = single_slot(st);
     const unsigned long long wbase = st->it_work_base;
-    if (feed_idle<SINGLE>(n_act)) return;
+    if (feed_idle(n_act)) return;
     __shared__ uint32_t pre[kSegs + 1], bas[kSegs], feed_ctr;
-    if (SINGLE && threadIdx.x == 0) feed_ctr = blockDim.x >> 6;
+    if (threadIdx.x == 0) feed_ctr = blockDim.x >> 6;
     if (threadIdx.x <= kSegs) pre[threadIdx.x] = st->in_prefix[threadIdx.x];
     if (threadIdx.x < kSegs) bas[threadIdx.x] = st->in_base[threadIdx.x];
     if constexpr (!SINGLE) stage_jobs(c, st, &jt);
     else stage_job0(c, slot0, &job0);   // stage_lds synchronises
     stage_lds<SINGLE>(c, lds, &sp);   // synchronises
-    TraversalFeed<SINGLE> f;
+    BlockFeed f;
     feed_first(f, n_act);
     LdsNodes src{};
     int nn = 0, sc = 0;
@@ -666,7 +656,7 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        if constexpr (SINGLE) feed_ctr = blockDim.x >> 6;
+        feed_ctr = blockDim.x >> 6;
         // The shadow rays of this iteration, segment by segment (windows of the shade pass).
         uint32_t acc = 0, next = 0;
         for (int s = 0; s < kSegs; ++s) {
@@ -686,10 +676,10 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
     }
     __syncthreads();
     const uint32_t n = pre[kSegs];
-    if (feed_idle<SINGLE>(n)) return;
+    if (feed_idle(n)) return;
     if constexpr (!SINGLE) stage_jobs(c, st, &jt, done_cnt, done_tag);
     stage_lds<SINGLE>(c, lds, &sp);
-    TraversalFeed<SINGLE> f;
+    BlockFeed f;
     feed_first(f, n);
     LdsNodes src{};
     int nn = 0;
