@@ -12,6 +12,12 @@ Prints one JSON line with:
                duration (stage-timed render with HIP events on the engine's stream), against 8 TB/s
   cpu_baseline the oracle (a C restatement of the same estimator) on the host cores, bounded sample
   quality      persistent == megakernel bitwise, zero overflows / invalid hits / non-finite pixels
+  pools        the same render with the path pool capped at 2^24 and 2^22 slots (3.1 and 0.8 GB of
+               path state instead of the work-sized 10.9 GB): Mrays/s and p50 at each
+  configs      the other benchmark workloads, each also on its own clock: heightfield 1280x720x64
+               (Mrays/s, p50/p99, speedup over the megakernel), the 64-job batch in one launch
+               (Mrays/s, speedup over the megakernel rendering the jobs one at a time) and the
+               serving loop (capacity, and Mrays/s with p50/p99 latency at a fixed Poisson rate)
 """
 import argparse
 import json
@@ -24,8 +30,17 @@ sys.path.insert(0, os.path.join(ROOT, "graphics-book-tutorials_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 METRIC = "Mrays/s at 1280x720x64spp depth 8 per scene; p50/p99 render latency"
-TUNING_KEYS = ("block_size", "pool_paths", "lds_kib", "refill", "leaf_batch", "dir_bins", "shade_block")
+TUNING_KEYS = ("block_size", "pool_paths", "lds_kib", "refill", "leaf_batch", "dir_bins", "shade_block",
+               "shade_sort")
 HBM_PEAK_GBS = 8000.0
+# Host threads of the CPU baseline: the CPUs of this process's affinity mask, at most the host share
+# of one GPU on a shared multi-GPU host (16).
+CPU_SHARE = 16
+# Capped pool sizes timed beside the work-sized pool, and renders per timing.
+POOL_CAPS = (1 << 24, 1 << 22)
+POOL_STEPS = 10
+POOL_SLOT_BYTES = 184   # two 64 B path-state buffers, the 8 B hit and a 48 B shadow-queue entry per slot
+HF_STEPS = 10
 # Device-memory traffic against the algorithmic bytes, from rocprofv3 FETCH_SIZE and WRITE_SIZE on this
 # workload (sphere_grid 1280x720x64, default tuning), each counter scaled by what it reads on a plain
 # copy kernel of known size (FETCH_SIZE x2.0, WRITE_SIZE x1.0); source and commands:
@@ -46,6 +61,38 @@ def percentile(v, p):
     return v[i] + (v[min(i + 1, len(v) - 1)] - v[i]) * f
 
 
+def cpu_description():
+    """The host CPU as /proc/cpuinfo describes it: model name, family/model/stepping, clock, logical
+    CPUs and sockets (a virtual machine may report a generic model name; the numbers identify it)."""
+    d = {"model": "", "family": None, "model_number": None, "stepping": None, "mhz": None, "logical_cpus": 0,
+         "sockets": 0, "flags_avx512": False}
+    phys = set()
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "processor":
+                d["logical_cpus"] += 1
+            elif k == "model name" and not d["model"]:
+                d["model"] = v
+            elif k == "cpu family" and d["family"] is None:
+                d["family"] = int(v)
+            elif k == "model" and d["model_number"] is None:
+                d["model_number"] = int(v)
+            elif k == "stepping" and d["stepping"] is None:
+                d["stepping"] = int(v) if v.isdigit() else v
+            elif k == "cpu MHz" and d["mhz"] is None:
+                d["mhz"] = float(v)
+            elif k == "physical id":
+                phys.add(v)
+            elif k == "flags" and not d["flags_avx512"]:
+                d["flags_avx512"] = "avx512f" in v.split()
+    except OSError:
+        pass
+    d["sockets"] = len(phys)
+    return d
+
+
 def cpu_baseline(scene, scene_name, w, h, spp_sample, gpu_sums, runs=3):
     """The oracle (a plain-C restatement of the same estimator) rendering the first spp_sample
     samples of the workload on the host cores: median wall time of `runs` renders. Its image is
@@ -54,7 +101,8 @@ def cpu_baseline(scene, scene_name, w, h, spp_sample, gpu_sums, runs=3):
     bitwise-equal pixels)."""
     import numpy as np
     import pyoracle
-    threads = max(1, min(16, os.cpu_count() or 1))
+    visible = len(os.sched_getaffinity(0))
+    threads = max(1, min(CPU_SHARE, visible))
     o = pyoracle.OScene(scene.arrays())
     times, sums, cnt = [], None, None
     for _ in range(runs):
@@ -66,22 +114,97 @@ def cpu_baseline(scene, scene_name, w, h, spp_sample, gpu_sums, runs=3):
     rgb_g, var_g = pyoracle.finalize(*gpu_sums, spp_sample)
     cmp = pyoracle.compare(rgb_g, var_g, spp_sample, rgb_c, var_c, spp_sample)
     same = (rgb_g.view(np.uint32).reshape(-1, 3) == rgb_c.view(np.uint32).reshape(-1, 3)).all(axis=1)
-    model = ""
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
+    cpu = cpu_description()
     base = {"value": (cnt[0] + cnt[1]) / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"{scene_name} {w}x{h}, samples 0..{spp_sample - 1} of 64, depth 8, seed 1234: {cnt[0]} "
                       f"extension + {cnt[1]} shadow rays, median of {runs} renders {dt:.3f} s "
-                      f"({', '.join(f'{t:.3f}' for t in times)}); {model}"}
+                      f"({', '.join(f'{t:.3f}' for t in times)}); {threads} threads of the {visible} CPUs in "
+                      f"this process's affinity mask (at most {CPU_SHARE}: the host share of one GPU); "
+                      f"{cpu['model']}",
+            "host_cpus_visible": visible, "cpu": cpu}
     gate = {"frac_within_3se": cmp["frac_within_3se"], "threshold": 0.999,
             "pass": cmp["frac_within_3se"] >= 0.999 and cmp["nonfinite"] == 0,
             "bitwise_pixel_fraction": float(same.mean()), "samples": spp_sample}
     return base, gate
+
+
+def other_configs(wfamd):
+    """The other benchmark workloads, each timed on its own (wfbench.py holds their protocol):
+    heightfield 1280x720x64 on the persistent engine (autotuned) beside the megakernel; the 64-job
+    batch in one launch beside the megakernel rendering the same views one at a time, every job
+    bitwise equal to its standalone render; the serving loop's capacity (every request at once) and
+    its latency and throughput at a fixed Poisson arrival rate."""
+    import numpy as np
+    import wfbench
+    out = {}
+    t_all = time.perf_counter()
+    # heightfield: 2.1 M triangles, BVH built on the host (outside the timed region).
+    cfg = wfbench.PROTOCOL["configs"]["2"]
+    w, h, spp = cfg["width"], cfg["height"], cfg["spp"]
+    seed, depth = wfbench.PROTOCOL["seed"], wfbench.PROTOCOL["max_depth"]
+    scene = wfamd.Scene(cfg["scene"], w, h)
+    tuned = wfamd.autotune([(scene, spp, seed, depth)], probe_spp=16)
+    pt = wfamd.params(spp=spp, seed=seed, max_depth=depth, **{k: getattr(tuned, k) for k in TUNING_KEYS})
+    r = wfamd.Renderer(scene, "persistent")
+    for _ in range(2):
+        r.reset()
+        r.run_passes(0, spp, pt)
+    lat, rays = [], 0
+    t0 = time.perf_counter()
+    for _ in range(HF_STEPS):
+        r.reset()
+        s = r.run_passes(0, spp, pt)
+        lat.append(s.render_ms)
+        rays += s.extension_rays + s.shadow_rays
+    el = time.perf_counter() - t0
+    sums_p = r.download_sums()
+    r.close()
+    mk = wfamd.Renderer(scene, "megakernel")
+    mk.reset()
+    mk.run_passes(0, spp, pt)
+    mk_lat = []
+    for _ in range(3):
+        mk.reset()
+        ms = mk.run_passes(0, spp, pt)
+        mk_lat.append(ms.render_ms)
+    same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(mk.download_sums(), sums_p))
+    mk.close()
+    mk_mrays = (ms.extension_rays + ms.shadow_rays) / (percentile(mk_lat, 0.5) * 1e3)
+    out["2"] = {"workload": "heightfield 1280x720 64spp depth 8 (2,093,058 triangles)",
+                "mrays_per_s": rays / el / 1e6, "ms_per_step": el / HF_STEPS * 1e3, "steps": HF_STEPS,
+                "p50_ms": percentile(lat, 0.5), "p99_ms": percentile(lat, 0.99),
+                "megakernel": {"mrays_per_s": mk_mrays, "p50_ms": percentile(mk_lat, 0.5)},
+                "speedup_vs_megakernel": percentile(mk_lat, 0.5) / percentile(lat, 0.5),
+                "persistent_equals_megakernel_bitwise": bool(same),
+                "queue_overflows": int(s.queue_overflows), "invalid_hits": int(s.invalid_hits),
+                "tuning": {k: getattr(tuned, k) for k in TUNING_KEYS}}
+    print(f"[bench] heightfield: {out['2']['ms_per_step']:.2f} ms ({out['2']['mrays_per_s']:.0f} Mrays/s), "
+          f"megakernel {out['2']['megakernel']['p50_ms']:.1f} ms, bitwise={same}", file=sys.stderr, flush=True)
+    # 64 jobs in one launch.
+    d = wfbench.bench_batch(dict(wfbench.PROTOCOL["configs"]["3"]), False)
+    out["3"] = {"workload": "64 jobs (scene of configs 0-2, orbit angle, 128x128..640x360 log-uniform, "
+                            "spp in {1,4,16,64}, depth 8, seed 2025) in one persistent launch",
+                **{k: d[k] for k in ("jobs", "pixels", "paths", "rays", "mrays_per_s", "megakernel_mrays_per_s",
+                                     "megakernel_one_by_one_ms", "speedup_vs_megakernel_one_by_one",
+                                     "jobs_bitwise_equal_standalone", "rays_equal", "queue_overflows",
+                                     "invalid_hits", "iterations", "tuning", "autotune_s")},
+                "batch_ms_p50": d["batch_render_ms"]["median"], "batch_ms_runs": d["batch_render_ms"]["runs"]}
+    print(f"[bench] batch: {out['3']['batch_ms_p50']:.2f} ms ({out['3']['mrays_per_s']:.0f} Mrays/s), "
+          f"{out['3']['speedup_vs_megakernel_one_by_one']:.1f}x the megakernel one by one, "
+          f"bitwise={out['3']['jobs_bitwise_equal_standalone']}", file=sys.stderr, flush=True)
+    # Serving loop.
+    d = wfbench.bench_serving(dict(wfbench.PROTOCOL["configs"]["4"]), False)
+    out["4"] = {"workload": "serving: 48 requests (scene of configs 0-2, orbit angle, 256x256..1920x1080 and "
+                            "4..256 spp log-uniform, depth 8, seed 4242), Poisson arrivals, 8 GiB budget",
+                **{k: d[k] for k in ("requests", "capacity_requests_per_s", "closed_loop_mrays_per_s",
+                                     "arrival_rate_per_s", "mrays_per_s", "latency_ms", "completed", "rejected",
+                                     "peak_in_flight", "alone_equals_mix_bitwise", "checked_alone_vs_mix")}}
+    print(f"[bench] serving: capacity {out['4']['capacity_requests_per_s']:.1f} req/s; at "
+          f"{out['4']['arrival_rate_per_s']:.0f} req/s: {out['4']['mrays_per_s']:.0f} Mrays/s, p50 "
+          f"{out['4']['latency_ms']['p50']:.1f} ms, p99 {out['4']['latency_ms']['p99']:.1f} ms",
+          file=sys.stderr, flush=True)
+    out["wall_s"] = time.perf_counter() - t_all
+    return out
 
 
 def main():
@@ -96,6 +219,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-spp", type=int, default=16, help="samples per pixel of the CPU baseline sample")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-pools", action="store_true", help="skip the renders at capped pool sizes")
+    ap.add_argument("--no-configs", action="store_true", help="skip the heightfield, batch and serving workloads")
     ap.add_argument("--pool-paths", type=int, default=0, help="persistent path slots (0: sized to the work)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run the plain-copy calibration kernel over BYTES (traffic-counter scale)")
@@ -205,6 +330,39 @@ def main():
                            for k, v in stages.items()}}
     r.close()
 
+    # The same render with the path pool capped below the work (fewer slots: more iterations).
+    pools = None
+    if not args.no_pools:
+        pools = {}
+        for cap in POOL_CAPS:
+            pt = wfamd.params(spp=spp, seed=seed, max_depth=depth,
+                              **{k: getattr(tuning, k) for k in TUNING_KEYS if k != "pool_paths"}, pool_paths=cap)
+            rp = wfamd.Renderer(scene, "persistent")
+            for _ in range(2):
+                rp.reset()
+                rp.run_passes(0, spp, pt)
+            pl, prays = [], 0
+            tp = time.perf_counter()
+            for _ in range(POOL_STEPS):
+                rp.reset()
+                s = rp.run_passes(0, spp, pt)
+                pl.append(s.render_ms)
+                prays += s.extension_rays + s.shadow_rays
+            pel = time.perf_counter() - tp
+            same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(rp.download_sums(), sums_p))
+            pools[str(cap)] = {"mrays_per_s": prays / pel / 1e6, "ms_per_step": pel / POOL_STEPS * 1e3,
+                               "p50_ms": percentile(pl, 0.5), "p99_ms": percentile(pl, 0.99),
+                               "iterations": int(s.iterations), "path_state_bytes": cap * POOL_SLOT_BYTES,
+                               "equals_work_sized_pool_bitwise": bool(same)}
+            rp.close()
+            print(f"[bench] pool {cap}: {pools[str(cap)]['ms_per_step']:.2f} ms per render "
+                  f"({pools[str(cap)]['mrays_per_s']:.0f} Mrays/s), {s.iterations} iterations, bitwise={same}",
+                  file=sys.stderr, flush=True)
+
+    configs = None
+    if not args.no_configs:
+        configs = other_configs(wfamd)
+
     cpu, cpu_gate = None, None
     if not args.no_cpu:
         # GPU side of the CPU quality gate: the same first cpu_spp samples on the persistent engine.
@@ -237,7 +395,7 @@ def main():
                     "nonfinite_pixels": int((~np.isfinite(rgb_p.reshape(-1, 3))).any(axis=1).sum()),
                     "nonfinite_var": int((~np.isfinite(var_p)).sum()),
                     "cpu_vs_persistent_within_3se": cpu_gate},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "pools": pools, "configs": configs,
     }
     print(json.dumps(out), flush=True)
 

@@ -35,7 +35,8 @@ inline std::string usage() {
     return "Usage: wf render --scene NAME --width W --height H --spp N --seed S --max-depth D\n"
            "       --mode {cpu,megakernel,wavefront,persistent} --report PATH --out PREFIX --threads N --reps N "
            "--warmup N\n"
-           "Scenes: cornell divergence trianglehv furnace cornell555 sphere_grid heightfield\n";
+           "Scenes: cornell divergence trianglehv furnace cornell555 sphere_grid heightfield\n"
+           "Also:  wf validate-report REPORT.json | wf imgdiff A.pfm B.pfm [--tol T]\n";
 }
 
 inline bool parse_int(const char* s, int& out) {
@@ -220,6 +221,75 @@ inline void write_pfm(const std::string& path, int w, int h, const std::vector<f
     if (!f) throw std::runtime_error("cannot write " + path);
     f << "PF\n" << w << " " << h << "\n-1.0\n";
     for (int y = h - 1; y >= 0; --y) f.write((const char*)(rgb.data() + (size_t)3 * w * y), (std::streamsize)(12 * w));
+}
+
+// Reads a PFM image into top-row-first RGB floats (src/image_io.cpp:188-250). "PF" holds three
+// channels, "Pf" one (returned as grey RGB); the scale's sign gives the byte order (negative:
+// little endian). The header is magic, width, height and scale separated by whitespace, then
+// exactly one whitespace byte before the rows, bottom row first. Errors name the problem.
+inline std::vector<float> read_pfm(const std::string& path, int* width, int* height) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("read_pfm: cannot open " + path);
+    std::string magic;
+    f >> magic;
+    if (magic != "PF" && magic != "Pf") throw std::runtime_error("read_pfm: not a PFM file");
+    const int channels = magic == "PF" ? 3 : 1;
+    long long w = 0, h = 0;
+    double scale = 0.0;
+    if (!(f >> w >> h) || w <= 0 || h <= 0 || w > (1 << 16) || h > (1 << 16))
+        throw std::runtime_error("read_pfm: invalid dimensions");
+    if (!(f >> scale) || scale == 0.0 || !std::isfinite(scale)) throw std::runtime_error("read_pfm: bad scale");
+    const int sep = f.get();
+    if (sep != '\n' && sep != '\r' && sep != ' ' && sep != '\t') throw std::runtime_error("read_pfm: bad header");
+    const bool big = scale > 0.0;
+    const size_t row_bytes = (size_t)w * channels * 4;
+    std::vector<unsigned char> row(row_bytes);
+    std::vector<float> rgb((size_t)3 * w * h);
+    for (long long y = h - 1; y >= 0; --y) {
+        f.read((char*)row.data(), (std::streamsize)row_bytes);
+        if (f.gcount() != (std::streamsize)row_bytes) throw std::runtime_error("read_pfm: truncated file");
+        for (long long x = 0; x < w; ++x)
+            for (int c = 0; c < 3; ++c) {
+                const unsigned char* b = row.data() + ((size_t)x * channels + (channels == 3 ? c : 0)) * 4;
+                const uint32_t u = big ? ((uint32_t)b[0] << 24 | (uint32_t)b[1] << 16 | (uint32_t)b[2] << 8 | b[3])
+                                       : ((uint32_t)b[3] << 24 | (uint32_t)b[2] << 16 | (uint32_t)b[1] << 8 | b[0]);
+                float v;
+                std::memcpy(&v, &u, 4);
+                rgb[((size_t)y * w + x) * 3 + c] = v;
+            }
+    }
+    *width = (int)w;
+    *height = (int)h;
+    return rgb;
+}
+
+// Difference of two images of equal size (tools/imgdiff.cpp): the largest relative difference
+// |a-b| / max(|a|,|b|) over channel pairs where that maximum exceeds 1e-20, and the relative
+// difference of the channel sums |sum a - sum b| / sum a (0 when sum a <= 1e-12). A non-finite
+// value must compare equal to its counterpart (the same infinity); a NaN never does.
+struct ImageDiff {
+    double max_abs_rel = 0.0;
+    double mean_rel = 0.0;
+    bool nonfinite_mismatch = false;
+};
+
+inline ImageDiff image_diff(const std::vector<float>& a, const std::vector<float>& b) {
+    if (a.size() != b.size()) throw std::invalid_argument("image_diff: sizes differ");
+    ImageDiff d;
+    double sa = 0.0, sb = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) {
+        const float x = a[i], y = b[i];
+        if (!std::isfinite(x) || !std::isfinite(y)) {
+            if (!(x == y)) d.nonfinite_mismatch = true;
+            continue;
+        }
+        sa += x;
+        sb += y;
+        const double m = std::max(std::fabs((double)x), std::fabs((double)y));
+        if (m > 1e-20) d.max_abs_rel = std::max(d.max_abs_rel, std::fabs((double)x - (double)y) / m);
+    }
+    if (sa > 1e-12) d.mean_rel = std::fabs(sa - sb) / sa;
+    return d;
 }
 
 }  // namespace wfcli

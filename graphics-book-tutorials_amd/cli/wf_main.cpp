@@ -11,6 +11,7 @@
 #include <memory>
 
 #include "wf_cli.h"
+#include "wf_json.h"
 #include "wfamd.h"
 
 using namespace wfcli;
@@ -39,9 +40,80 @@ int mode_id(const std::string& m) {
     return WF_MODE_PERSISTENT;
 }
 
+// `wf validate-report FILE`: checks a report.json against schema version 1 (src/report.cpp:348-475).
+// Exit 0 and "valid", or exit 1 and the first problem; 2 when the file cannot be read.
+int validate_report_command(int argc, const char* argv[]) {
+    if (argc != 3) {
+        std::cerr << "Usage: wf validate-report REPORT.json\n";
+        return 2;
+    }
+    std::ifstream f(argv[2], std::ios::binary);
+    if (!f) {
+        std::cerr << "Error: cannot open " << argv[2] << "\n";
+        return 2;
+    }
+    std::ostringstream text;
+    text << f.rdbuf();
+    const std::string problem = validate_report_json(text.str());
+    if (problem.empty()) {
+        std::printf("valid\n");
+        return 0;
+    }
+    std::printf("invalid: %s\n", problem.c_str());
+    return 1;
+}
+
+// `wf imgdiff A.pfm B.pfm [--tol T]` (tools/imgdiff.cpp): prints max_abs_rel and mean_rel; exit 0
+// when max_abs_rel <= T (default 1e-5), 1 when above, 2 on an error (unreadable file, different
+// sizes, a non-finite value without an equal counterpart).
+int imgdiff_command(int argc, const char* argv[]) {
+    if (argc < 4) {
+        std::cerr << "Usage: wf imgdiff A.pfm B.pfm [--tol T]\n";
+        return 2;
+    }
+    double tol = 1e-5;
+    for (int i = 4; i < argc; ++i) {
+        const std::string a = argv[i];
+        if (a == "--tol" && i + 1 < argc) {
+            char* end = nullptr;
+            tol = std::strtod(argv[++i], &end);
+            if (!end || *end != '\0' || !(tol >= 0.0)) {
+                std::cerr << "Error: --tol: not a non-negative number\n";
+                return 2;
+            }
+        } else {
+            std::cerr << "Error: unknown option " << a << "\n";
+            return 2;
+        }
+    }
+    int w1 = 0, h1 = 0, w2 = 0, h2 = 0;
+    const std::vector<float> a = read_pfm(argv[2], &w1, &h1);
+    const std::vector<float> b = read_pfm(argv[3], &w2, &h2);
+    if (w1 != w2 || h1 != h2) {
+        std::cerr << "Error: image sizes differ (" << w1 << "x" << h1 << " and " << w2 << "x" << h2 << ")\n";
+        return 2;
+    }
+    const ImageDiff d = image_diff(a, b);
+    if (d.nonfinite_mismatch) {
+        std::cerr << "Error: a non-finite value differs between the images\n";
+        return 2;
+    }
+    std::printf("max_abs_rel=%.17g\nmean_rel=%.17g\n", d.max_abs_rel, d.mean_rel);
+    return d.max_abs_rel <= tol ? 0 : 1;
+}
+
 }  // namespace
 
 int main(int argc, const char* argv[]) {
+    if (argc >= 2 && std::string(argv[1]) == "validate-report") return validate_report_command(argc, argv);
+    if (argc >= 2 && std::string(argv[1]) == "imgdiff") {
+        try {
+            return imgdiff_command(argc, argv);
+        } catch (const std::exception& e) {
+            std::cerr << "Error: " << e.what() << std::endl;
+            return 2;
+        }
+    }
     try {
         Config c = parse_args(argc, argv);
         wf_scene* scene = nullptr;

@@ -71,6 +71,7 @@ struct PersistentTuning {
     int leaf_batch = 0;    // parked leaves that trigger a wave's leaf-test round (0 -> 16)
     int dir_bins = 0;      // survivor grouping per block round: 1 none, 8 direction octants (0 -> 1)
     int shade_block = 0;   // threads per shade workgroup (= compaction round): 128..1024 (0 -> 512)
+    int shade_sort = 0;    // single-job shade rounds grouped by material: 1 off, 2 on (0 -> default)
 };
 
 struct PersistentResult {
@@ -225,8 +226,11 @@ private:
 // plan of a batch's workgroups).
 LdsPlan plan_lds(const int* n_nodes, const int* n_prims, int n, long long budget);
 
-// Picks block size, pool size and LDS budget for a workload by timing short renders of it.
+// Picks block size, traversal scheduling, shade block, survivor grouping, LDS budget and queue
+// capacity (path slots) for a workload by timing renders of it. state_budget: the most device bytes
+// of path state the pool may take (kBytesPerSlot a slot; 0 = no limit).
 PersistentTuning autotune_persistent(const PersistentJob* jobs, int n_jobs, hipStream_t stream, int probe_spp,
-                                     std::vector<std::pair<PersistentTuning, double>>* trials);
+                                     std::vector<std::pair<PersistentTuning, double>>* trials,
+                                     uint64_t state_budget = 0);
 
 }  // namespace wf::gpu

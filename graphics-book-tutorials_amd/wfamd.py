@@ -34,7 +34,8 @@ class RenderParams(C.Structure):
                 ("count_rays", C.c_int), ("stage_timing", C.c_int), ("queue_capacity", C.c_int),
                 ("trace_bounces", C.c_int), ("block_size", C.c_int), ("pool_paths", C.c_int),
                 ("scan_tile", C.c_int), ("lds_nodes", C.c_int), ("lds_kib", C.c_int), ("refill", C.c_int),
-                ("leaf_batch", C.c_int), ("dir_bins", C.c_int), ("shade_block", C.c_int), ("graph_iters", C.c_int)]
+                ("leaf_batch", C.c_int), ("dir_bins", C.c_int), ("shade_block", C.c_int), ("graph_iters", C.c_int),
+                ("shade_sort", C.c_int)]
 
 
 class RenderStats(C.Structure):
@@ -95,7 +96,8 @@ class BatchStats(C.Structure):
                 ("invalid_hits", C.c_int64), ("queue_overflows", C.c_int64), ("iterations", C.c_int64),
                 ("graph_launches", C.c_int64), ("block_size", C.c_int), ("pool_paths", C.c_int),
                 ("lds_kib", C.c_int), ("refill", C.c_int), ("leaf_batch", C.c_int),
-                ("dir_bins", C.c_int), ("shade_block", C.c_int), ("device_bytes", C.c_uint64),
+                ("dir_bins", C.c_int), ("shade_block", C.c_int), ("shade_sort", C.c_int),
+                ("device_bytes", C.c_uint64),
                 ("stage_ms", C.c_double * 8), ("stage_items", C.c_int64 * 8), ("stage_launches", C.c_int64 * 8),
                 ("stage_bytes", C.c_int64 * 8), ("stage_threads", C.c_int64 * 8)]
 
@@ -124,7 +126,7 @@ EXPORTS = [
     "wf_render_cpu_sums", "wf_finalize", "wf_query_device", "wf_scan_workspace_create",
     "wf_scan_workspace_destroy", "wf_scan_exclusive_u32", "wf_compact_u32", "wf_compact_u32_noalloc",
     "wf_test_queue_push",
-    "wf_batch_render", "wf_autotune", "wf_server_create", "wf_server_destroy", "wf_server_submit",
+    "wf_batch_render", "wf_autotune", "wf_autotune_budget", "wf_server_create", "wf_server_destroy", "wf_server_submit",
     "wf_server_step", "wf_server_poll", "wf_server_stats_get", "wf_test_sincos", "wf_test_sincos_grid_hash",
     "wf_test_intersect", "wf_test_slab", "wf_test_divu32", "wf_test_lds_plan", "wf_test_trace_paths", "wf_copy_probe",
 ]
@@ -169,6 +171,7 @@ def lib():
         "wf_test_queue_push": (I, [P, I, P, C.c_uint32, P, P, P]),
         "wf_batch_render": (I, [C.POINTER(Job), I, C.POINTER(RenderParams), C.POINTER(BatchStats)]),
         "wf_autotune": (I, [C.POINTER(Job), I, I, C.POINTER(RenderParams)]),
+        "wf_autotune_budget": (I, [C.POINTER(Job), I, I, U64, C.POINTER(RenderParams)]),
         "wf_server_create": (I, [U64, C.POINTER(RenderParams), C.POINTER(P)]),
         "wf_server_destroy": (None, [P]),
         "wf_server_submit": (I, [P, C.POINTER(Job), C.POINTER(C.c_int64)]),
@@ -209,10 +212,10 @@ def _iptr(a):
 
 def params(spp=64, seed=1234, max_depth=8, threads=0, stage_timing=False, queue_capacity=0, trace_bounces=False,
            block_size=0, pool_paths=0, scan_tile=0, lds_nodes=-1, lds_kib=0, refill=0, leaf_batch=0,
-           dir_bins=0, shade_block=0, graph_iters=0):
+           dir_bins=0, shade_block=0, graph_iters=0, shade_sort=0):
     return RenderParams(spp, seed, max_depth, threads, 0, int(stage_timing), queue_capacity, int(trace_bounces),
                         block_size, pool_paths, scan_tile, lds_nodes, lds_kib, refill, leaf_batch, dir_bins,
-                        shade_block, graph_iters)
+                        shade_block, graph_iters, shade_sort)
 
 
 class Scene:
@@ -494,10 +497,15 @@ def batch_render(specs, tuning=None):
     return outs, st
 
 
-def autotune(specs, probe_spp=2):
+def autotune(specs, probe_spp=2, state_budget=None):
+    """Tuning for a batch (wf_autotune); state_budget: the most device bytes of path state its pool
+    may take (wf_autotune_budget)."""
     jobs, _ = make_jobs(specs)
     p = params()
-    _check(lib().wf_autotune(jobs, len(specs), probe_spp, C.byref(p)))
+    if state_budget is None:
+        _check(lib().wf_autotune(jobs, len(specs), probe_spp, C.byref(p)))
+    else:
+        _check(lib().wf_autotune_budget(jobs, len(specs), probe_spp, int(state_budget), C.byref(p)))
     return p
 
 

@@ -18,7 +18,7 @@ version 1 (src/bench.cpp:198-330): conditions megakernel and wavefront, the same
 Config 3 renders 64 jobs of random scene, orbit angle, resolution and spp in one persistent
 launch (autotuned) and compares with rendering the same views one by one on the megakernel; every
 job's image must equal its standalone render bit for bit.
-Config 4 is the serving loop: Poisson arrivals at about 75% of the measured capacity, requests of
+Config 4 is the serving loop: Poisson arrivals at a fixed rate (60 requests/s), requests of
 random scene, orbit angle, resolution and spp under a device-memory budget; it reports Mrays/s and
 p50/p99 latency (arrival to result) and checks that requests give the same bits alone and in the mix.
 Every random choice comes from a seeded generator, and every pinned value is in PROTOCOL.
@@ -36,7 +36,8 @@ sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 import wfamd  # noqa: E402
 
 # Persistent-engine tuning fields (wf_render_params) the autotuner chooses.
-TUNING_KEYS = ("block_size", "pool_paths", "lds_kib", "refill", "leaf_batch", "dir_bins", "shade_block")
+TUNING_KEYS = ("block_size", "pool_paths", "lds_kib", "refill", "leaf_batch", "dir_bins", "shade_block",
+               "shade_sort")
 
 # Pinned protocol (the reference pins its values in bench/protocol.json).
 PROTOCOL = {
@@ -54,7 +55,7 @@ PROTOCOL = {
         "3": {"jobs": 64, "job_seed": 2025, "scenes": ["cornell555", "sphere_grid", "heightfield"],
               "width": [128, 640], "height": [128, 360], "spp": [1, 4, 16, 64], "reps": 5, "warmup": 1},
         "4": {"requests": 48, "request_seed": 4242, "scenes": ["cornell555", "sphere_grid", "heightfield"],
-              "width": [256, 1920], "height": [256, 1080], "spp": [4, 256], "load": 0.75,
+              "width": [256, 1920], "height": [256, 1080], "spp": [4, 256], "rate_per_s": 60.0,
               "budget_bytes": 8 << 30, "check_smallest": 6},
     },
     "quick": {"width": 96, "height": 54, "spp": 4, "reps": 3, "warmup": 1, "jobs": 8, "requests": 8},
@@ -115,6 +116,69 @@ def apply_verdict(r):
     if r["speedup_meets_target"] and r["vram_within_budget"] and not reasons:
         r["verdict"] = "hypothesis_supported"
     return r
+
+
+BENCH_KEYS = ("scene_name", "scene_hash", "width", "height", "spp", "max_depth", "reps", "warmup", "seed",
+              "device_name", "driver_version", "cuda_runtime_version", "conditions", "quality", "speedup",
+              "speedup_meets_target", "vram_within_budget", "verdict", "reasons")
+CONDITION_KEYS = ("mode", "run_ms", "min", "median", "p10", "p90", "max", "spread_rel", "samples_per_second",
+                  "rays_per_second", "peak_vram_bytes", "queue_high_water", "queue_overflows", "invalid_hits",
+                  "nonfinite_pixels")
+
+
+def validate_bench_doc(doc):
+    """Checks a bench document (a parsed JSON object) against schema version 1 and the verdict rule
+    (src/bench.cpp:369-441). Returns "" when valid, else the name of the first offending key or
+    check ("verdict" when the verdict does not follow from the document's own fields)."""
+    def num(v):
+        return isinstance(v, (int, float)) and not isinstance(v, bool)
+    if not isinstance(doc, dict):
+        return "root"
+    if not num(doc.get("schema_version")) or int(doc["schema_version"]) != PROTOCOL["schema_version"]:
+        return "schema_version"
+    for k in BENCH_KEYS:
+        if k not in doc:
+            return k
+    h = doc["scene_hash"]
+    if not isinstance(h, str) or len(h) != 16 or any(c not in "0123456789abcdef" for c in h):
+        return "scene_hash"
+    if not num(doc["speedup"]):
+        return "speedup"
+    for k in ("speedup_meets_target", "vram_within_budget"):
+        if not isinstance(doc[k], bool):
+            return k
+    conds = doc["conditions"]
+    if not isinstance(conds, list) or len(conds) != 2:
+        return "conditions"
+    if not isinstance(doc["quality"], list):
+        return "quality"
+    if not isinstance(doc["reasons"], list):
+        return "reasons"
+    if conds[0].get("mode") != "megakernel" or conds[1].get("mode") != "wavefront":
+        return "mode"
+    for c in conds:
+        for k in CONDITION_KEYS:
+            if k not in c:
+                return k
+        if not isinstance(c["run_ms"], list) or not c["run_ms"] or not all(num(e) for e in c["run_ms"]):
+            return "run_ms"
+        if not (c["min"] <= c["p10"] <= c["median"] <= c["p90"] <= c["max"]):
+            return "stats"
+    for q in doc["quality"]:
+        for k in ("name", "pass", "value", "threshold", "detail"):
+            if k not in q:
+                return k
+        if not isinstance(q["pass"], bool):
+            return "pass"
+    qual_ok = all(q["pass"] for q in doc["quality"])
+    valid_ok = all(c["queue_overflows"] == 0 and c["invalid_hits"] == 0 and c["nonfinite_pixels"] == 0 for c in conds)
+    speedup_ok = doc["speedup"] >= PROTOCOL["speedup_target"]
+    all_ok = speedup_ok and qual_ok and doc["vram_within_budget"] and valid_ok and not doc["reasons"]
+    if doc["verdict"] == "hypothesis_supported":
+        return "" if all_ok else "verdict"
+    if doc["verdict"] == "hypothesis_weakened":
+        return "verdict" if all_ok else ""
+    return "verdict"
 
 
 def nonfinite_pixels(rgb):
@@ -351,7 +415,10 @@ def bench_serving(cfg, quick):
     lat0, span0, st0, srv0, _, _ = serve(views, base, cfg["budget_bytes"], None, tuning)
     srv0.close()
     capacity = n / span0
-    rate = cfg["load"] * capacity
+    # A fixed arrival rate (PROTOCOL), or a fraction of the capacity just measured.
+    rate = cfg["rate_per_s"] if "rate_per_s" in cfg else cfg["load"] * capacity
+    if quick:
+        rate = 0.75 * capacity
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n)).tolist()
     lat, span, st, srv, ids, peak = serve(views, base, cfg["budget_bytes"], arrivals, tuning)
     # Alone vs in the mix: the smallest requests rendered standalone on the megakernel.
@@ -371,7 +438,7 @@ def bench_serving(cfg, quick):
     return {"config": 4, "requests": n, "budget_bytes": cfg["budget_bytes"],
             "capacity_requests_per_s": capacity, "closed_loop_makespan_s": span0,
             "closed_loop_mrays_per_s": (st0["extension_rays"] + st0["shadow_rays"]) / span0 / 1e6,
-            "arrival_rate_per_s": rate, "load": cfg["load"], "makespan_s": span,
+            "arrival_rate_per_s": rate, "load": rate / capacity, "makespan_s": span,
             "mrays_per_s": rays / span / 1e6,
             "latency_ms": {"p50": percentile(lat, 0.5), "p99": percentile(lat, 0.99), "max": max(lat),
                            "min": min(lat)},

@@ -59,6 +59,8 @@ typedef struct wf_render_params {
     int graph_iters;      /* persistent: pool iterations per captured graph launch (0 = max_depth + 2
                            * when every sample fits the pool, else 16; 8 in the serving loop;
                            * rounded up to even) */
+    int shade_sort;       /* persistent, one job: each shade round regrouped by material before
+                           * shading, 1 off, 2 on (0 = default) */
 } wf_render_params;
 
 /* RenderStats, include/wf/renderer.hpp:23-37. stage_ms: extend, classify, shade_lambert,
@@ -256,7 +258,7 @@ typedef struct wf_batch_stats {
     uint64_t extension_rays, shadow_rays;
     int64_t invalid_hits, queue_overflows;
     int64_t iterations, graph_launches;
-    int block_size, pool_paths, lds_kib, refill, leaf_batch, dir_bins, shade_block;
+    int block_size, pool_paths, lds_kib, refill, leaf_batch, dir_bins, shade_block, shade_sort;
     uint64_t device_bytes;
     /* With tuning->stage_timing: per-stage time, items, launches and algorithmic device bytes in
      * wf_render_stats' order (extend, -, shade, -, -, shadow, -, other); otherwise zero. */
@@ -270,9 +272,16 @@ typedef struct wf_batch_stats {
 int wf_batch_render(const wf_job* jobs, int n_jobs, const wf_render_params* tuning, wf_batch_stats* stats);
 /* Autotuner: a coordinate search over short renders of the batch (probe_spp samples per job):
  * block size, refill threshold x leaf batch, shade block, survivor grouping, LDS budget. The three
- * fastest candidates and the default tuning are then timed on the whole batch, and the fastest of
- * them is written into *tuning (pool_paths 0: sized to the work). Renders into the jobs' films. */
+ * fastest candidates and the default tuning are then timed on the whole batch; the fastest of them
+ * is timed again at smaller queue capacities (2^24 and 2^22 path slots), and the fastest of all is
+ * written into *tuning (pool_paths 0: sized to the work). Renders into the jobs' films. */
 int wf_autotune(const wf_job* jobs, int n_jobs, int probe_spp, wf_render_params* tuning);
+/* The same search with the path pool's device memory limited to state_budget_bytes (184 bytes a
+ * slot; 0 = no limit): every candidate's queue capacity fits it, so pool_paths in *tuning does.
+ * WF_ERR_INVALID_ARGUMENT when the budget holds fewer than 1024 slots. No reference counterpart
+ * (the reference has no autotuner). */
+int wf_autotune_budget(const wf_job* jobs, int n_jobs, int probe_spp, uint64_t state_budget_bytes,
+                       wf_render_params* tuning);
 
 /* ---- serving loop: continuous batching under a device-memory budget (no reference counterpart) ----
  * A request renders in consecutive sample ranges of about one path pool each, resolved into its

@@ -6,12 +6,15 @@
 //   png PATH W H             writes a W x H PNG of the test pattern (x*7 + y*13 + c*29) mod 256
 //   pfm PATH W H             writes a W x H PFM of the test pattern (x + 1000 y + 0.25 c)
 //   tonemap V...             the 8-bit tone-mapped value of each V
+//   readpfm PATH             width, height and the top-row-first floats' bits of a PFM, or {"exception": ...}
+//   json TEXT                {"ok": true, "kind": K} when TEXT parses as one JSON value, else {"error": ...}
 #include <cstdio>
 #include <cstdlib>
 #include <string>
 #include <vector>
 
 #include "wf_cli.h"
+#include "wf_json.h"
 
 using namespace wfcli;
 
@@ -68,6 +71,23 @@ int main(int argc, char** argv) {
             std::printf("{\"bytes\": [");
             for (size_t i = 0; i < t.size(); ++i) std::printf("%s%u", i ? ", " : "", (unsigned)t[i]);
             std::printf("]}\n");
+        } else if (cmd == "readpfm") {
+            int w = 0, h = 0;
+            const std::vector<float> px = read_pfm(argv[2], &w, &h);
+            std::printf("{\"width\": %d, \"height\": %d, \"bits\": [", w, h);
+            for (size_t i = 0; i < px.size(); ++i) {
+                uint32_t u;
+                std::memcpy(&u, &px[i], 4);
+                std::printf("%s%u", i ? ", " : "", u);
+            }
+            std::printf("]}\n");
+        } else if (cmd == "json") {
+            try {
+                const JValue v = json_parse(argv[2]);
+                std::printf("{\"ok\": true, \"kind\": %d}\n", (int)v.kind);
+            } catch (const JsonError& e) {
+                std::printf("{\"error\": %s}\n", jstr(e.what()).c_str());
+            }
         } else {
             return 2;
         }

@@ -181,6 +181,74 @@ def test_apply_verdict_quality_vram_and_validity_rules(wf):
     assert len(set(v(r)["reasons"])) == 6
 
 
+def _bench_doc(mk=13.0, wavefront=10.0):
+    """A whole schema-v1 bench document with its verdict applied (src/bench.cpp:198-330)."""
+    import wfbench
+    def cond(m, runs):
+        c = wfbench.finalize_condition({"mode": m, "run_ms": list(runs)}, 8, 4, 2)
+        c.update({"rays_per_second": 1e6, "peak_vram_bytes": "1000", "queue_high_water": [1, 2, 3, 4, 5],
+                  "queue_overflows": 0, "invalid_hits": 0, "nonfinite_pixels": 0})
+        return c
+    doc = {"schema_version": 1, "scene_name": "cornell", "scene_hash": "2ec1deb9ff15147d", "width": 8, "height": 4,
+           "spp": 2, "max_depth": 8, "reps": 3, "warmup": 1, "seed": 1234, "device_name": "d", "driver_version": "1",
+           "cuda_runtime_version": "1", "conditions": [cond("megakernel", (mk - 1, mk, mk + 1)),
+                                                        cond("wavefront", (wavefront - 1, wavefront, wavefront + 1))],
+           "quality": [{"name": n, "pass": True, "value": 1.0, "threshold": 0.999, "detail": ""}
+                       for n in ("bitwise", "cpu_vs_megakernel", "cpu_vs_wavefront")]}
+    return wfbench.apply_verdict(doc)
+
+
+def test_bench_document_validator(wf):
+    """validate_bench_json (src/bench.cpp:369-441): keys, types, ordered statistics and a verdict
+    that follows from the document's own fields."""
+    import copy
+    import wfbench
+    val = wfbench.validate_bench_doc
+    for mk in (13.0, 12.9):
+        d = _bench_doc(mk)
+        assert val(d) == "" and val(json.loads(json.dumps(d))) == ""
+    d = _bench_doc()
+    assert d["verdict"] == "hypothesis_supported"
+    for k in wfbench.BENCH_KEYS:
+        e = copy.deepcopy(d)
+        del e[k]
+        assert val(e) == k
+    for k in wfbench.CONDITION_KEYS:
+        e = copy.deepcopy(d)
+        del e["conditions"][1][k]
+        assert val(e) in (k, "mode"), k
+    bad = [("schema_version", 2, "schema_version"), ("scene_hash", "2EC1DEB9FF15147D", "scene_hash"),
+           ("scene_hash", "2ec1deb9ff15147", "scene_hash"), ("speedup", "1.3", "speedup"),
+           ("speedup_meets_target", 1, "speedup_meets_target"), ("vram_within_budget", None, "vram_within_budget"),
+           ("reasons", "", "reasons"), ("quality", {}, "quality"), ("verdict", "maybe", "verdict"),
+           ("verdict", "hypothesis_weakened", "verdict")]
+    for k, v, want in bad:
+        e = copy.deepcopy(d)
+        e[k] = v
+        assert val(e) == want, (k, v)
+    e = copy.deepcopy(d)
+    e["conditions"] = e["conditions"][::-1]
+    assert val(e) == "mode"
+    e = copy.deepcopy(d)
+    e["conditions"][0]["p10"] = e["conditions"][0]["median"] + 0.5
+    assert val(e) == "stats"
+    e = copy.deepcopy(d)
+    e["conditions"][0]["run_ms"] = []
+    assert val(e) == "run_ms"
+    e = copy.deepcopy(d)
+    e["quality"][0]["pass"] = "yes"
+    assert val(e) == "pass"
+    # A supported verdict next to a failed check, an overflow or a reason is inconsistent.
+    for mutate in (lambda x: x["quality"][1].__setitem__("pass", False),
+                   lambda x: x["conditions"][0].__setitem__("invalid_hits", 1),
+                   lambda x: x["reasons"].append("r"), lambda x: x.__setitem__("speedup", 1.2)):
+        e = copy.deepcopy(d)
+        mutate(e)
+        assert val(e) == "verdict"
+        assert val(wfbench.apply_verdict(e)) == ""      # re-applying the rule makes it consistent
+    assert val([]) == "root"
+
+
 # ---------------------------------------------------------------------------------------------
 # images (src/image_io.cpp)
 # ---------------------------------------------------------------------------------------------
@@ -329,3 +397,242 @@ def test_wf_render_rejects_a_bad_flag_with_usage(tmp_path):
     r = subprocess.run([WF_BIN, "render", "--mode", "cpu", "--width", "8", "--height", "8", "--spp", "1",
                         "--report", str(tmp_path / "no" / "such" / "dir" / "r.json")], capture_output=True, text=True)
     assert r.returncode == 1 and "report" in r.stderr
+
+
+# ---------------------------------------------------------------------------------------------
+# report.json validator (src/report.cpp:348-475; tests/test_report.cpp:401-520)
+# ---------------------------------------------------------------------------------------------
+def validate(path):
+    r = subprocess.run([WF_BIN, "validate-report", str(path)], capture_output=True, text=True)
+    return r.returncode, r.stdout.strip()
+
+
+@pytest.fixture(scope="module")
+def cpu_report(tmp_path_factory):
+    d = tmp_path_factory.mktemp("report")
+    rep = d / "report.json"
+    r = subprocess.run([WF_BIN, "render", "--scene", "cornell", "--width", "8", "--height", "6", "--spp", "1",
+                        "--mode", "cpu", "--threads", "2", "--reps", "3", "--report", str(rep), "--out", str(d / "img")],
+                       capture_output=True, text=True, cwd=ROOT)
+    assert r.returncode == 0, r.stderr
+    return json.load(open(rep))
+
+
+REQUIRED = [("schema_version",), ("git_revision",), ("scene",), ("scene", "name"), ("scene", "hash"), ("config",),
+            ("config", "width"), ("config", "height"), ("config", "spp"), ("config", "seed"), ("config", "max_depth"),
+            ("config", "mode"), ("device",), ("device", "name"), ("device", "driver_version"),
+            ("device", "cuda_runtime_version"), ("device", "cuda_driver_version"), ("device", "compute_capability"),
+            ("device", "total_memory_bytes"), ("timings_ms",), ("timings_ms", "runs"), ("timings_ms", "min"),
+            ("timings_ms", "median"), ("timings_ms", "p10"), ("timings_ms", "p90"), ("timings_ms", "max"),
+            ("throughput",), ("throughput", "samples_per_second"), ("throughput", "rays_per_second"),
+            ("peak_vram_bytes",), ("queue_high_water",), ("queue_high_water", "lambert"), ("queue_high_water", "metal"),
+            ("queue_high_water", "dielectric"), ("queue_high_water", "shadow"), ("queue_high_water", "active"),
+            ("outputs",), ("outputs", "png"), ("outputs", "pfm"), ("outputs", "pfm_hash"), ("validity",),
+            ("validity", "nonfinite_pixels"), ("validity", "queue_overflows"), ("validity", "invalid_hits")]
+
+
+def mutated(doc, path, fn):
+    import copy
+    d = copy.deepcopy(doc)
+    node = d
+    for k in path[:-1]:
+        node = node[k]
+    fn(node, path[-1])
+    return d
+
+
+def check_rejects(tmp_path, doc, what):
+    p = tmp_path / "m.json"
+    p.write_text(json.dumps(doc))
+    rc, out = validate(p)
+    assert rc == 1 and out.startswith("invalid:"), f"{what} accepted: {out}"
+    return out
+
+
+def test_validator_accepts_the_cli_report_and_its_reserialisation(cpu_report, tmp_path):
+    p = tmp_path / "r.json"
+    p.write_text(json.dumps(cpu_report, indent=None))
+    assert validate(p) == (0, "valid")
+    p.write_text(json.dumps({**cpu_report, "extra_key": [1, {"x": None}]}))   # keys beyond the schema are allowed
+    assert validate(p) == (0, "valid")
+
+
+def test_validator_rejects_each_missing_required_key(cpu_report, tmp_path):
+    for path in REQUIRED:
+        out = check_rejects(tmp_path, mutated(cpu_report, path, lambda n, k: n.pop(k)), f"missing {path}")
+        assert path[-1] in out, (path, out)
+
+
+def test_validator_rejects_each_value_of_the_wrong_type(cpu_report, tmp_path):
+    # A boolean is the wrong type at every schema position.
+    for path in REQUIRED:
+        check_rejects(tmp_path, mutated(cpu_report, path, lambda n, k: n.__setitem__(k, True)), f"bool at {path}")
+
+
+def test_validator_rejects_swapped_number_and_string(cpu_report, tmp_path):
+    check_rejects(tmp_path, mutated(cpu_report, ("schema_version",), lambda n, k: n.__setitem__(k, "1")), "string")
+    check_rejects(tmp_path, mutated(cpu_report, ("config", "width"), lambda n, k: n.__setitem__(k, "320")), "width")
+    check_rejects(tmp_path, mutated(cpu_report, ("scene", "name"), lambda n, k: n.__setitem__(k, 5)), "name")
+    check_rejects(tmp_path, mutated(cpu_report, ("timings_ms", "runs"), lambda n, k: n[k].__setitem__(0, "fast")),
+                  "run")
+    check_rejects(tmp_path, mutated(cpu_report, ("throughput", "rays_per_second"),
+                                    lambda n, k: n.__setitem__(k, "many")), "rays_per_second")
+
+
+def test_validator_schema_version_runs_order_and_hex_fields(cpu_report, tmp_path):
+    for v in (2, 0):
+        check_rejects(tmp_path, mutated(cpu_report, ("schema_version",), lambda n, k: n.__setitem__(k, v)), v)
+    check_rejects(tmp_path, mutated(cpu_report, ("timings_ms", "runs"), lambda n, k: n.__setitem__(k, [])), "empty")
+    t = cpu_report["timings_ms"]
+    check_rejects(tmp_path, mutated(cpu_report, ("timings_ms", "min"), lambda n, k: n.__setitem__(k, t["median"] + 1)),
+                  "min > median")
+    check_rejects(tmp_path, mutated(cpu_report, ("timings_ms", "median"), lambda n, k: n.__setitem__(k, t["max"] + 1)),
+                  "median > max")
+    for bad in ("0123456789abcde", "0123456789abcdef0", "0123456789abcdeg"):
+        check_rejects(tmp_path, mutated(cpu_report, ("scene", "hash"), lambda n, k: n.__setitem__(k, bad)), bad)
+        check_rejects(tmp_path, mutated(cpu_report, ("outputs", "pfm_hash"), lambda n, k: n.__setitem__(k, bad)), bad)
+    # null where the schema allows it
+    p = tmp_path / "ok.json"
+    p.write_text(json.dumps(mutated(mutated(cpu_report, ("git_revision",), lambda n, k: n.__setitem__(k, None)),
+                                    ("throughput", "rays_per_second"), lambda n, k: n.__setitem__(k, None))))
+    assert validate(p) == (0, "valid")
+
+
+def test_validator_rejects_text_that_is_not_one_json_object(cpu_report, tmp_path):
+    p = tmp_path / "t.json"
+    good = json.dumps(cpu_report)
+    for text in ("", "[]", "1", good + " x", good[:-1], good.replace(":", "=", 1), '{"a": 1,}'):
+        p.write_text(text)
+        rc, out = validate(p)
+        assert rc == 1 and out.startswith("invalid:"), (text[:40], out)
+    rc, _ = validate(tmp_path / "absent.json")
+    assert rc == 2
+
+
+@pytest.mark.parametrize("text,ok", [
+    ('{"a": [1, -0.5, 2e10, 1E-3, true, false, null, "x"]}', True), ('"\\u00e9\\ud83d\\ude00\\n\\t\\/"', True),
+    ("  [ ]  ", True), ("01", False), ("1.", False), ("-", False), (".5", False), ("1e", False), ("[1,]", False),
+    ('{"a" 1}', False), ("tru", False), ('"\\x"', False), ('"\\ud83d"', False), ('"\\ude00"', False),
+    ('"a\nb"', False), ("[1] [2]", False), ("{'a': 1}", False), ("[" * 300 + "]" * 300, False),
+])
+def test_json_reader_is_strict(units, text, ok):
+    r = units("json", text)
+    assert ("ok" in r) == ok, (text[:30], r)
+
+
+# ---------------------------------------------------------------------------------------------
+# read_pfm and imgdiff (src/image_io.cpp:188-250, tools/imgdiff.cpp)
+# ---------------------------------------------------------------------------------------------
+def write_pfm_py(path, img, big_endian=False, magic=b"PF", sep=b"\n"):
+    h, w = img.shape[:2]
+    scale = b"1.0" if big_endian else b"-1.0"
+    rows = img[::-1].astype(">f4" if big_endian else "<f4")
+    open(path, "wb").write(magic + b"\n%d %d\n" % (w, h) + scale + sep + rows.tobytes())
+
+
+def test_read_pfm_round_trips_and_reads_both_byte_orders(units, tmp_path):
+    rng = np.random.default_rng(7)
+    img = rng.standard_normal((5, 7, 3)).astype(np.float32)
+    img[0, 0] = [np.inf, -np.inf, 0.0]
+    for big in (False, True):
+        p = tmp_path / f"i{big}.pfm"
+        write_pfm_py(p, img, big_endian=big)
+        r = units("readpfm", p)
+        assert (r["width"], r["height"]) == (7, 5)
+        assert np.array_equal(np.array(r["bits"], np.uint32), img.ravel().view(np.uint32)), big
+    units("pfm", tmp_path / "c.pfm", 4, 3)               # the CLI's own writer
+    r = units("readpfm", tmp_path / "c.pfm")
+    want = np.array([[[x + 1000.0 * y + 0.25 * c for c in range(3)] for x in range(4)] for y in range(3)], np.float32)
+    assert np.array_equal(np.array(r["bits"], np.uint32), want.ravel().view(np.uint32))
+    grey = rng.random((2, 3)).astype(np.float32)
+    p = tmp_path / "g.pfm"
+    open(p, "wb").write(b"Pf\n3 2\n-1.0\n" + grey[::-1].astype("<f4").tobytes())
+    r = units("readpfm", p)
+    assert np.array_equal(np.array(r["bits"], np.uint32), np.repeat(grey.ravel(), 3).view(np.uint32))
+
+
+def test_read_pfm_rejects_malformed_files(units, tmp_path):
+    img = np.zeros((2, 2, 3), np.float32)
+    cases = {"magic": (b"P6\n2 2\n-1.0\n", "not a PFM"), "dims": (b"PF\n0 2\n-1.0\n", "dimensions"),
+             "sep": (b"PF\n2 2\n-1.0x", "header"), "scale": (b"PF\n2 2\n0.0\n", "scale")}
+    for name, (head, msg) in cases.items():
+        p = tmp_path / f"{name}.pfm"
+        open(p, "wb").write(head + img.astype("<f4").tobytes())
+        r = units("readpfm", p)
+        assert msg in r.get("exception", ""), (name, r)
+    p = tmp_path / "short.pfm"
+    open(p, "wb").write(b"PF\n2 2\n-1.0\n" + img.astype("<f4").tobytes()[:-1])
+    assert "truncated" in units("readpfm", p)["exception"]
+    assert "cannot open" in units("readpfm", tmp_path / "absent.pfm")["exception"]
+
+
+def imgdiff(a, b, *extra):
+    r = subprocess.run([WF_BIN, "imgdiff", str(a), str(b), *map(str, extra)], capture_output=True, text=True)
+    vals = dict(line.split("=") for line in r.stdout.split())
+    return r.returncode, {k: float(v) for k, v in vals.items()}
+
+
+def test_imgdiff_statistics_and_exit_status(tmp_path):
+    rng = np.random.default_rng(3)
+    a = rng.random((6, 5, 3)).astype(np.float32) + 0.1
+    b = a.copy()
+    b[2, 3, 1] *= np.float32(1.001)
+    b[0, 0, 0] = 0.0
+    pa, pb = tmp_path / "a.pfm", tmp_path / "b.pfm"
+    write_pfm_py(pa, a)
+    write_pfm_py(pb, b)
+    assert imgdiff(pa, pa) == (0, {"max_abs_rel": 0.0, "mean_rel": 0.0})
+    rc, v = imgdiff(pa, pb)
+    x, y = a.astype(np.float64).ravel(), b.astype(np.float64).ravel()
+    m = np.maximum(np.abs(x), np.abs(y))
+    want_max = float(np.max(np.abs(x - y)[m > 1e-20] / m[m > 1e-20]))
+    assert rc == 1 and v["max_abs_rel"] == want_max == 1.0
+    assert v["mean_rel"] == pytest.approx(abs(x.sum() - y.sum()) / x.sum(), rel=1e-12)
+    assert imgdiff(pa, pb, "--tol", "1.0")[0] == 0
+    c = a.copy()
+    c[1, 1, 1] *= np.float32(1.0 + 1e-3)
+    write_pfm_py(pb, c)
+    assert imgdiff(pa, pb)[0] == 1 and imgdiff(pa, pb, "--tol", "1e-2")[0] == 0
+    write_pfm_py(pb, a[:5])
+    assert imgdiff(pa, pb)[0] == 2                       # sizes differ
+    n = a.copy()
+    n[0, 0, 0] = np.nan
+    write_pfm_py(pb, n)
+    assert imgdiff(pa, pb)[0] == 2 and imgdiff(pb, pb)[0] == 2   # a NaN never matches
+    i = a.copy()
+    i[0, 0, 0] = np.inf
+    write_pfm_py(pb, i)
+    assert imgdiff(pb, pb)[0] == 0 and imgdiff(pa, pb)[0] == 2
+    assert imgdiff(pa, pa, "--bogus")[0] == 2
+
+
+# ---------------------------------------------------------------------------------------------
+# GPU arms through the CLI at the full Cornell-box configuration (320x180, 16 spp, depth 8)
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.gpu
+def test_wf_render_gpu_modes_report_and_pfm_equal_the_oracle(wf, oracle_ref, tmp_path):
+    w, h, spp = 320, 180, 16
+    scene = wf.Scene("cornell555", w, h)
+    (orgb, olum, olum2), ocnt = oracle_ref(scene, spp)
+    want_rgb, _ = wf.finalize(orgb, olum, olum2, spp)
+    pfms = []
+    for mode in ("megakernel", "wavefront", "persistent"):
+        rep, out = tmp_path / f"{mode}.json", tmp_path / mode
+        print(f"wf render cornell555 {w}x{h}x{spp} --mode {mode}", flush=True)
+        r = subprocess.run([WF_BIN, "render", "--scene", "cornell555", "--width", str(w), "--height", str(h), "--spp",
+                            str(spp), "--mode", mode, "--reps", "2", "--warmup", "1", "--report", str(rep), "--out",
+                            str(out)], capture_output=True, text=True, cwd=ROOT, timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert validate(rep) == (0, "valid"), mode
+        j = json.load(open(rep))
+        assert j["config"]["mode"] == mode and j["device"]["name"] != "cpu"
+        assert j["validity"] == {"nonfinite_pixels": 0, "queue_overflows": 0, "invalid_hits": 0}
+        assert (j["mi355x"]["extension_rays"], j["mi355x"]["shadow_rays"]) == (ocnt[0], ocnt[1]), mode
+        data = open(str(out) + ".pfm", "rb").read()
+        head = b"PF\n%d %d\n-1.0\n" % (w, h)
+        assert data.startswith(head)
+        img = np.frombuffer(data[len(head):], "<f4").reshape(h, w, 3)[::-1].copy()
+        assert_bitwise(img.ravel(), np.asarray(want_rgb).ravel(), f"{mode} PFM vs oracle")
+        pfms.append(str(out) + ".pfm")
+    for p in pfms[1:]:
+        assert imgdiff(pfms[0], p, "--tol", "0") == (0, {"max_abs_rel": 0.0, "mean_rel": 0.0})

@@ -499,3 +499,71 @@ def test_sky_light_nee_is_unbiased_for_a_convex_lambert_sphere(wf):
     rgb, var, _ = wf.render_cpu(s, wf.params(spp=64, max_depth=4, threads=8))
     m, se = mean_se(rgb, var, 64)
     assert abs(m - 0.5) <= 4.5 * se + 1e-4 and se < 0.005
+
+
+# ---------------------------------------------------------------------------------------------
+# heightfield parameters (6-octave fBm, amplitude 0.25, lacunarity 2, gain 0.5)
+# ---------------------------------------------------------------------------------------------
+def _lowbias32(x):
+    x = np.asarray(x, np.uint64) & 0xFFFFFFFF
+    x ^= x >> 16
+    x = (x * 0x7FEB352D) & 0xFFFFFFFF
+    x ^= x >> 15
+    x = (x * 0x846CA68B) & 0xFFFFFFFF
+    x ^= x >> 16
+    return x
+
+
+def _grad_noise(x, z, seed):
+    """2D gradient noise: a unit gradient per lattice corner from hashing the corner and the
+    seed, dotted with the offset to the corner, blended by the quintic fade; scaled by sqrt(2)."""
+    fx0, fz0 = np.floor(x), np.floor(z)
+    ix, iz = fx0.astype(np.int64), fz0.astype(np.int64)
+    fx, fz = x - fx0, z - fz0
+    g = []
+    for dx, dz in ((0, 0), (1, 0), (0, 1), (1, 1)):
+        cx = ((ix + dx) & 0xFFFFFFFF) * 0x8DA6B343 & 0xFFFFFFFF
+        cz = ((iz + dz) & 0xFFFFFFFF) * 0xD8163841 & 0xFFFFFFFF
+        h = _lowbias32(_lowbias32(cx ^ cz) ^ np.uint64(seed & 0xFFFFFFFF))
+        a = (h >> 8).astype(np.float64) * (1.0 / 16777216.0) * 6.283185307179586
+        g.append(np.cos(a) * (fx - dx) + np.sin(a) * (fz - dz))
+    fade = lambda t: t * t * t * (t * (t * 6.0 - 15.0) + 10.0)
+    u, v = fade(fx), fade(fz)
+    a = g[0] + (g[1] - g[0]) * u
+    b = g[2] + (g[3] - g[2]) * u
+    return (a + (b - a) * v) * 1.4142135623730951
+
+
+def _fbm(x, z, octaves=6, amp=0.25, lacunarity=2.0, gain=0.5, f0=0.4, seed=7):
+    h = np.zeros_like(x)
+    f = f0
+    for o in range(octaves):
+        h += amp * _grad_noise(x * f, z * f, (seed + o * 0x9E3779B9) & 0xFFFFFFFF)
+        amp *= gain
+        f *= lacunarity
+    return h
+
+
+def test_heightfield_heights_are_the_stated_fbm(wf, po):
+    """Vertex heights equal 6-octave fBm (amplitude 0.25, lacunarity 2, gain 0.5, base frequency
+    0.4, seed 7) of gradient noise, evaluated here in numpy on a sample of vertices; each parameter
+    changed by a little moves the heights, so the comparison pins all of them. The corners of
+    [-5, 5]^2 sit on the lattice of every octave, where gradient noise vanishes."""
+    for v in (0, 1, 12345, 0xFFFFFFFF):
+        assert int(_lowbias32(v)) == po.hash_u32(v)
+    s = wf.Scene("heightfield", 64, 36)
+    vert = s.arrays()["vertices"].reshape(1024, 1024, 3)
+    rng = np.random.default_rng(5)
+    i, j = rng.integers(0, 1024, 20000), rng.integers(0, 1024, 20000)
+    x = -5.0 + 10.0 * i / 1023.0
+    z = -5.0 + 10.0 * j / 1023.0
+    assert np.array_equal(vert[i, j, 0], x.astype(np.float32)) and np.array_equal(vert[i, j, 2], z.astype(np.float32))
+    y = vert[i, j, 1]
+    want = _fbm(x, z).astype(np.float32)
+    assert np.array_equal(y, want), int((y != want).sum())
+    for k in ((0, 0), (0, 1023), (1023, 0), (1023, 1023)):
+        assert vert[k[0], k[1], 1] == 0.0
+    for change in ({"octaves": 5}, {"octaves": 7}, {"amp": 0.26}, {"lacunarity": 2.01}, {"gain": 0.51},
+                   {"f0": 0.41}, {"seed": 8}):
+        other = _fbm(x, z, **change).astype(np.float32)
+        assert (other != y).mean() > 0.5, change

@@ -122,3 +122,33 @@ def test_serving_configuration_poisson_arrivals_under_a_binding_budget(wf, po, o
             assert_bitwise(rgb, orgb, f"request {k} rgb vs oracle")
             assert_bitwise(var, ovar, f"request {k} var vs oracle")
     srv.close()
+
+
+def test_autotuner_queue_capacity_within_a_state_budget(wf, oracle_ref):
+    """The autotuner's queue-capacity search: under a path-state budget every candidate pool fits
+    it, the chosen tuning renders the same bits as the megakernel (and the oracle), and without a
+    budget the work-sized pool competes with 2^22 slots."""
+    s = wf.Scene("sphere_grid", 320, 180)
+    spp = 96                                   # 5,529,600 paths: more than 2^22
+    slot = 184
+    for budget in (None, 256 << 20, 2 << 30):
+        print(f"[autotune] sphere_grid 320x180x{spp} state budget {budget}", flush=True)
+        t = wf.autotune([(s, spp, 1234, 8)], probe_spp=4, state_budget=budget)
+        pool = t.pool_paths if t.pool_paths > 0 else (320 * 180 * spp + 1023) // 1024 * 1024
+        print(f"[autotune]   block {t.block_size} pool {t.pool_paths} ({pool * slot / 2**20:.0f} MiB) "
+              f"lds {t.lds_kib} refill {t.refill} leaf_batch {t.leaf_batch} shade_block {t.shade_block}", flush=True)
+        assert pool * slot <= (budget or 1 << 62)
+        if budget == 256 << 20:
+            assert t.pool_paths == 1 << 20             # the largest power of two within the budget
+        else:
+            assert t.pool_paths in (0, 1 << 22)        # work-sized, or the smaller candidate
+        r = wf.Renderer(s, "persistent")
+        r.reset()
+        st = r.run_passes(0, spp, t)
+        got = r.download_sums()
+        r.close()
+        from conftest import assert_equals_oracle
+        assert_equals_oracle(got, (st.extension_rays, st.shadow_rays, st.invalid_hits), oracle_ref(s, spp),
+                             f"autotuned (budget {budget})")
+    with pytest.raises(wf.InvalidArgument):
+        wf.autotune([(s, 4, 1234, 8)], probe_spp=2, state_budget=1000 * slot)

@@ -131,6 +131,9 @@ def test_wavefront_equals_megakernel_and_oracle_trace(wf, po, scenes, oracle_ref
     otrace, _, _ = o.wavefront_pass(0, 1234, 8)
     assert trace0 == otrace, f"{name}: GPU {trace0[:3]} oracle {otrace[:3]}"
     assert trace0[0][0] == w * h
+    # Queue high-water marks of sample 0 (kernel_update_counters, src/gpu/wavefront.cu): the
+    # largest lambert, metal, dielectric and shadow queue and active count over its bounces.
+    assert list(st.queue_high_water) == [max(t[k] for t in otrace) for k in (1, 2, 3, 4, 0)], name
 
 
 @pytest.mark.parametrize("name,w,h", SMALL)
@@ -218,6 +221,15 @@ def test_overflow_is_counted_and_renderer_survives(wf, po, scenes, oracle_ref):
     r = wf.Renderer(s, "wavefront")
     _, _, st = r.render(wf.params(spp=2, queue_capacity=100))
     assert st.queue_overflows > 0
+    # High-water marks are capped at the capacity. Which paths a full queue drops depends on the
+    # order of the pushes, so only the first classify (before any drop) is fixed: its capped
+    # material-queue counts, as the oracle's restatement with the same cap records them, are lower
+    # bounds; the shadow queue is filled by whichever Lambert paths were kept.
+    o = po.OScene(s.arrays())
+    first = o.wavefront_pass(0, 1234, 8, queue_capacity=100)[0][0]
+    hw = list(st.queue_high_water)
+    assert hw[4] == 64 * 36 and all(first[k + 1] <= hw[k] <= 100 for k in range(3)), (hw, first)
+    assert 0 < hw[3] <= 100 and max(hw[:3]) == 100
     rgb, var, st2 = r.render(wf.params(spp=2))
     r.close()
     assert st2.queue_overflows == 0
