@@ -30,8 +30,7 @@ sys.path.insert(0, os.path.join(ROOT, "graphics-book-tutorials_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 METRIC = "Mrays/s at 1280x720x64spp depth 8 per scene; p50/p99 render latency"
-TUNING_KEYS = ("block_size", "pool_paths", "lds_kib", "refill", "leaf_batch", "dir_bins", "shade_block",
-               "shade_sort")
+TUNING_KEYS = ("block_size", "pool_paths", "lds_kib", "refill", "leaf_batch", "dir_bins", "shade_block")
 HBM_PEAK_GBS = 8000.0
 # Host threads of the CPU baseline: the CPUs of this process's affinity mask, at most the host share
 # of one GPU on a shared multi-GPU host (16).

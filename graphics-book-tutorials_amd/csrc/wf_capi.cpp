@@ -68,7 +68,6 @@ wf::RenderParams to_params(const wf_render_params* p) {
     r.leaf_batch = p->leaf_batch;
     r.dir_bins = p->dir_bins;
     r.shade_block = p->shade_block;
-    r.shade_sort = p->shade_sort;
     r.graph_iters = p->graph_iters;
     return r;
 }
@@ -718,7 +717,6 @@ wf::gpu::PersistentTuning to_tuning(const wf_render_params* p) {
         t.leaf_batch = p->leaf_batch;
         t.dir_bins = p->dir_bins;
         t.shade_block = p->shade_block;
-        t.shade_sort = p->shade_sort;
         t.graph_iters = p->graph_iters;
     }
     return t;
@@ -821,7 +819,6 @@ extern "C" int wf_batch_render(const wf_job* jobs, int n_jobs, const wf_render_p
             stats->leaf_batch = t.leaf_batch;
             stats->dir_bins = t.dir_bins;
             stats->shade_block = t.shade_block;
-            stats->shade_sort = t.shade_sort;
             stats->device_bytes = eng.bytes() + b.scenes.bytes();
             const double ms8[8] = {stages.extend,           stages.classify, stages.shade_lambert, stages.shade_metal,
                                    stages.shade_dielectric, stages.shadow,   stages.compact,       stages.other};
@@ -856,7 +853,6 @@ extern "C" int wf_autotune_budget(const wf_job* jobs, int n_jobs, int probe_spp,
         tuning->leaf_batch = best.leaf_batch;
         tuning->dir_bins = best.dir_bins;
         tuning->shade_block = best.shade_block;
-        tuning->shade_sort = best.shade_sort;
         return (int)WF_OK;
     });
 }

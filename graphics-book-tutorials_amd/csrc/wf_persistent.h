@@ -71,7 +71,6 @@ struct PersistentTuning {
     int leaf_batch = 0;    // parked leaves that trigger a wave's leaf-test round (0 -> 16)
     int dir_bins = 0;      // survivor grouping per block round: 1 none, 8 direction octants (0 -> 1)
     int shade_block = 0;   // threads per shade workgroup (= compaction round): 128..1024 (0 -> 512)
-    int shade_sort = 0;    // single-job shade rounds grouped by material: 1 off, 2 on (0 -> default)
 };
 
 struct PersistentResult {

@@ -29,8 +29,6 @@ constexpr int kDefaultRefill = 32;
 constexpr int kDefaultLeafBatch = 16;
 constexpr int kDefaultDirBins = 1;
 constexpr int kDefaultShadeBlock = 512;
-// Shade items of a round grouped by material before shading (single-job kernels): 1 off, 2 on.
-constexpr int kDefaultShadeSort = 1;
 // Allowance for the extend and shadow kernels' static LDS (the plan, segment tables and, in the
 // multi-job kernels, the job and scene tables and finished counts: under 7.5 KiB at gfx950).
 constexpr int kLdsStatic = 8 * 1024;
@@ -499,34 +497,12 @@ __device__ __forceinline__ void shade_load(const PCtx& c, const PathBuf& in, con
     }
 }
 
-// Material grouping of a shade round (SORT): the items of a round are dealt to the round's
-// threads grouped by what the shading will do with them, so that a wave runs one material's code
-// instead of all three with most lanes masked. Buckets: (miss, Lambert, metal, dielectric,
-// emissive) x (survivor, new path), then the items past the end. The order of items inside the
-// shading does not matter for the result: every path's state, random numbers and sample slot are
-// its own, and survivors and shadow rays go wherever their reserved slots are.
-constexpr int kShadeBuckets = 11;
-__device__ __forceinline__ int shade_bucket(const SceneView& s, float2 hv, bool is_new) {
-    const int prim = __float_as_int(hv.y);
-    const int cls = prim < 0 ? 0 : 1 + s.materials[s.prims[prim].material].type;
-    return cls * 2 + (is_new ? 1 : 0);
-}
-
-// Dynamic LDS of a grouped shade round of B items: survivor state o, d, t, l (float4 [B] each),
-// then the hits (float2 [B]) and the item indices (uint32 [B]).
-__host__ __device__ constexpr size_t shade_sort_lds(int B) { return (size_t)B * (4 * 16 + 8 + 4); }
-
 // Classify + shade of items [0, it_act). Survivors are compacted into `out`; slots for survivors
 // and shadow rays are reserved once per block round. Inside a round the survivors are grouped by
 // the octant of their new direction (dir_bins = 8), each group in item order, so the waves of the
-// next extend see rays of nearby pixels heading the same way. With SORT (single-job kernels) each
-// round's items are first regrouped by material through LDS (shade_bucket).
-template <bool SINGLE, bool SORT = false>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) void pw_shade_kernel(PCtx c, PathBuf in,
-                                                                                              PathBuf out) {
-    static_assert(!SORT || SINGLE, "material grouping is for the single-job kernels");
-    extern __shared__ float4 srec[];   // SORT only: the round's records (shade_sort_lds)
-    __shared__ uint32_t bucket_cnt[kShadeBuckets];
+// next extend see rays of nearby pixels heading the same way.
+template <bool SINGLE>
+__global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, PathBuf out) {
     __shared__ uint32_t cnt[2][10][16];   // per round parity: 8 direction bins, shadow, finished
     __shared__ uint32_t bases[2][2];
     __shared__ uint32_t pre[kSegs + 1], bas[kSegs], win[kSegs];
@@ -546,66 +522,15 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) void 
     if (threadIdx.x <= kSegs) pre[threadIdx.x] = st->in_prefix[threadIdx.x];
     if (threadIdx.x < kSegs) bas[threadIdx.x] = st->in_base[threadIdx.x];
     if (threadIdx.x == 0) seg_windows(n, blockDim.x, win);
-    if (threadIdx.x < kShadeBuckets) bucket_cnt[threadIdx.x] = 0u;
     __syncthreads();
     int round = 0;
     for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n; b0 += stride, round ^= 1) {
         const uint32_t seg = (b0 / blockDim.x) % kSegs;
-        uint32_t item = b0 + threadIdx.x;
+        const uint32_t item = b0 + threadIdx.x;
         // Loading the next round's inputs ahead (a software pipeline) measured slower at the
         // default 512-thread blocks: the extra live registers cost more occupancy than it hid.
         ShadeIn cur{};
         shade_load(c, in, pre, bas, item, n, n_live, &cur);
-        if constexpr (SORT) {
-            // Bucket of this thread's item; slots in the round's record list are reserved per
-            // wave and bucket (one LDS atomic per non-empty bucket), then the records are
-            // exchanged through LDS and thread t shades record t.
-            const uint32_t B = blockDim.x;
-            float4* ro = srec;
-            float4* rd = srec + B;
-            float4* rt = srec + 2 * B;
-            float4* rl = srec + 3 * B;
-            float2* rh = reinterpret_cast<float2*>(srec + 4 * B);
-            uint32_t* ri = reinterpret_cast<uint32_t*>(rh + B);
-            const int bk = item < n ? shade_bucket(c.scene0, cur.hv, item >= n_live) : kShadeBuckets - 1;
-            uint64_t mine = 0;
-            uint32_t woff = 0;
-            for (int b = 0; b < kShadeBuckets; ++b) {
-                const uint64_t m = __ballot(bk == b);
-                if (m == 0) continue;
-                uint32_t off = 0;
-                if (lane_id() == 0) off = atomicAdd(&bucket_cnt[b], (uint32_t)__popcll(m));
-                off = (uint32_t)__shfl((int)off, 0);
-                if (bk == b) {
-                    mine = m;
-                    woff = off;
-                }
-            }
-            __syncthreads();
-            uint32_t pos = woff + mbcnt64(mine);
-            for (int b = 0; b < bk; ++b) pos += bucket_cnt[b];
-            ri[pos] = item < n ? item : kIdle;
-            rh[pos] = cur.hv;
-            if (item < n_live) {
-                ro[pos] = cur.o;
-                rd[pos] = cur.d;
-                rt[pos] = cur.t;
-                rl[pos] = cur.l;
-            }
-            __syncthreads();
-            // Every thread has read the bucket counts; the next round adds to them only after
-            // this round's reservation barriers.
-            if (threadIdx.x < kShadeBuckets) bucket_cnt[threadIdx.x] = 0u;
-            const uint32_t me = threadIdx.x;
-            item = ri[me];
-            cur.hv = rh[me];
-            if (item < n_live) {
-                cur.o = ro[me];
-                cur.d = rd[me];
-                cur.t = rt[me];
-                cur.l = rl[me];
-            }
-        }
         bool live = false, fin = false;
         int has = 0;
         uint32_t jid = 0, sidx = 0, sc = 0;
@@ -866,7 +791,6 @@ struct PersistentEngine::Impl {
     int leaf_batch = kDefaultLeafBatch;
     int dir_bins = kDefaultDirBins;
     int shade_block = kDefaultShadeBlock;
-    int shade_sort = kDefaultShadeSort;
     std::vector<void*> pool_allocs;
     PState* st = nullptr;
     PState* st_host = nullptr;       // pinned mirror
@@ -901,14 +825,13 @@ struct PersistentEngine::Impl {
         bool single = false;
         int single_scene = -1;
         uint64_t scene_gen = 0;
-        int block = 0, refill = 0, leaf_batch = 0, dir_bins = 0, shade_block = 0, shade_sort = 0, cap = 0, iters = 0;
+        int block = 0, refill = 0, leaf_batch = 0, dir_bins = 0, shade_block = 0, cap = 0, iters = 0;
         size_t lds_bytes = 0;
         const void* ctr = nullptr;
         bool operator==(const GraphKey& o) const {
             return single == o.single && single_scene == o.single_scene && scene_gen == o.scene_gen &&
                    block == o.block && refill == o.refill &&
-                   leaf_batch == o.leaf_batch && dir_bins == o.dir_bins && shade_block == o.shade_block &&
-                   shade_sort == o.shade_sort && cap == o.cap && iters == o.iters && lds_bytes == o.lds_bytes &&
+                   leaf_batch == o.leaf_batch && dir_bins == o.dir_bins && shade_block == o.shade_block && cap == o.cap && iters == o.iters && lds_bytes == o.lds_bytes &&
                    ctr == o.ctr;
         }
     } graph_key;
@@ -968,9 +891,7 @@ struct PersistentEngine::Impl {
 struct PersistentEngine::LaunchPlan {
     PCtx c;
     bool single;
-    bool sort;          // single-job shade with material grouping
     size_t lds_bytes;
-    size_t shade_lds;   // dynamic LDS of the shade kernel
     int gx, gs, gw;
 };
 
@@ -1003,8 +924,6 @@ void PersistentEngine::ensure_pool(const PersistentTuning& t) {
     if (m.shade_block != 128 && m.shade_block != 256 && m.shade_block != 512 && m.shade_block != 1024)
         throw std::runtime_error("persistent: shade block size must be 128, 256, 512 or 1024");
     if (m.dir_bins != 1 && m.dir_bins != 8) throw std::runtime_error("persistent: direction bins must be 1 or 8");
-    m.shade_sort = t.shade_sort > 0 ? t.shade_sort : kDefaultShadeSort;
-    if (m.shade_sort != 1 && m.shade_sort != 2) throw std::runtime_error("persistent: shade sort must be 1 (off) or 2 (on)");
     m.ctx.refill = m.refill;
     m.ctx.leaf_batch = m.leaf_batch;
     m.ctx.dir_bins = m.dir_bins;
@@ -1312,20 +1231,14 @@ PersistentEngine::LaunchPlan PersistentEngine::make_plan(bool single) {
     }
     L.lds_bytes = (size_t)plan->total_nodes * sizeof(Node) + (size_t)plan->total_prims * sizeof(float4);
     const void* fx = single ? (const void*)pw_extend_kernel<true> : (const void*)pw_extend_kernel<false>;
-    L.sort = single && m.shade_sort == 2;
-    L.shade_lds = L.sort ? shade_sort_lds(m.shade_block) : 0;
-    const void* fs = L.sort ? (const void*)pw_shade_kernel<true, true>
-                            : single ? (const void*)pw_shade_kernel<true> : (const void*)pw_shade_kernel<false>;
-    if (L.shade_lds > 64 * 1024)
-        check(hipFuncSetAttribute(fs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.shade_lds),
-              "hipFuncSetAttribute");
+    const void* fs = single ? (const void*)pw_shade_kernel<true> : (const void*)pw_shade_kernel<false>;
     const void* fw = single ? (const void*)pw_shadow_kernel<true> : (const void*)pw_shadow_kernel<false>;
     if (L.lds_bytes > 64 * 1024)
         for (const void* fn : {fx, fw})
             check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.lds_bytes),
                   "hipFuncSetAttribute");
     L.gx = grid_for(fx, m.block, L.lds_bytes, m.cus);
-    L.gs = grid_for(fs, m.shade_block, L.shade_lds, m.cus);
+    L.gs = grid_for(fs, m.shade_block, 0, m.cus);
     L.gw = grid_for(fw, m.block, L.lds_bytes, m.cus);
     return L;
 }
@@ -1345,8 +1258,7 @@ void PersistentEngine::launch_iteration(const LaunchPlan& L, int par, hipEvent_t
     if (L.single) pw_extend_kernel<true><<<L.gx, m.block, L.lds_bytes, st>>>(c, in);
     else pw_extend_kernel<false><<<L.gx, m.block, L.lds_bytes, st>>>(c, in);
     if (ev) check(hipEventRecord(ev[2], st), "hipEventRecord");
-    if (L.sort) pw_shade_kernel<true, true><<<L.gs, m.shade_block, L.shade_lds, st>>>(c, in, out);
-    else if (L.single) pw_shade_kernel<true><<<L.gs, m.shade_block, 0, st>>>(c, in, out);
+    if (L.single) pw_shade_kernel<true><<<L.gs, m.shade_block, 0, st>>>(c, in, out);
     else pw_shade_kernel<false><<<L.gs, m.shade_block, 0, st>>>(c, in, out);
     if (ev) check(hipEventRecord(ev[3], st), "hipEventRecord");
     if (L.single) pw_shadow_kernel<true><<<L.gw, m.block, L.lds_bytes, st>>>(c, out);
@@ -1377,7 +1289,6 @@ void PersistentEngine::run(int launches) {
     k.leaf_batch = m.leaf_batch;
     k.dir_bins = m.dir_bins;
     k.shade_block = m.shade_block;
-    k.shade_sort = m.shade_sort;
     k.cap = cap_;
     k.iters = m.graph_iters;
     k.ctr = m.ctx.ctr;
@@ -1712,15 +1623,6 @@ PersistentTuning autotune_persistent(const PersistentJob* jobs, int n_jobs, hipS
         t.dir_bins = nb;
         time(t);
     }
-    if (n_jobs == 1) {   // material grouping of shade rounds (single-job kernels only)
-        base = best;
-        for (int ss : {1, 2}) {
-            if (ss == (base.shade_sort > 0 ? base.shade_sort : kDefaultShadeSort)) continue;
-            PersistentTuning t = base;
-            t.shade_sort = ss;
-            time(t);
-        }
-    }
     // Stage 3: LDS budget.
     base = best;
     for (int kib : {32, 40, 48, 64, 128}) {
@@ -1736,9 +1638,7 @@ PersistentTuning autotune_persistent(const PersistentJob* jobs, int n_jobs, hipS
     auto same = [](const PersistentTuning& x, const PersistentTuning& y) {
         return x.block_size == y.block_size && x.pool_paths == y.pool_paths && x.lds_kib == y.lds_kib &&
                x.graph_iters == y.graph_iters && x.refill == y.refill && x.leaf_batch == y.leaf_batch &&
-               x.dir_bins == y.dir_bins && x.shade_block == y.shade_block &&
-               (x.shade_sort > 0 ? x.shade_sort : kDefaultShadeSort) ==
-                   (y.shade_sort > 0 ? y.shade_sort : kDefaultShadeSort);
+               x.dir_bins == y.dir_bins && x.shade_block == y.shade_block;
     };
     std::stable_sort(seen.begin(), seen.end(), [](const auto& x, const auto& y) { return x.second < y.second; });
     std::vector<PersistentTuning> finalists;

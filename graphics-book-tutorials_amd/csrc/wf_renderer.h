@@ -29,7 +29,6 @@ struct RenderParams {
     int leaf_batch = 0;          // persistent: parked leaves per leaf-test round
     int dir_bins = 0;            // persistent: survivor grouping (1 or 8 direction octants)
     int shade_block = 0;         // persistent: threads per shade workgroup
-    int shade_sort = 0;          // persistent, one job: shade rounds grouped by material (1 off, 2 on)
     int graph_iters = 0;         // persistent: pool iterations per captured graph launch
     int lds_nodes = -1;          // nodes staged in LDS per workgroup (-1 = default)
 };

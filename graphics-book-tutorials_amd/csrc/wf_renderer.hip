@@ -159,7 +159,6 @@ void GpuRenderer::run_passes(int first_sample, int count, const RenderParams& p,
             tune.leaf_batch = p.leaf_batch;
             tune.dir_bins = p.dir_bins;
             tune.shade_block = p.shade_block;
-            tune.shade_sort = p.shade_sort;
             tune.graph_iters = p.graph_iters;
             gpu::PersistentResult pr;
             m.pe->render(&job, 1, tune, m.d_ctr, m.stream, &pr, p.stage_timing ? &st : nullptr);

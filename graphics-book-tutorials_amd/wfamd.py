@@ -34,8 +34,7 @@ class RenderParams(C.Structure):
                 ("count_rays", C.c_int), ("stage_timing", C.c_int), ("queue_capacity", C.c_int),
                 ("trace_bounces", C.c_int), ("block_size", C.c_int), ("pool_paths", C.c_int),
                 ("scan_tile", C.c_int), ("lds_nodes", C.c_int), ("lds_kib", C.c_int), ("refill", C.c_int),
-                ("leaf_batch", C.c_int), ("dir_bins", C.c_int), ("shade_block", C.c_int), ("graph_iters", C.c_int),
-                ("shade_sort", C.c_int)]
+                ("leaf_batch", C.c_int), ("dir_bins", C.c_int), ("shade_block", C.c_int), ("graph_iters", C.c_int)]
 
 
 class RenderStats(C.Structure):
@@ -96,7 +95,7 @@ class BatchStats(C.Structure):
                 ("invalid_hits", C.c_int64), ("queue_overflows", C.c_int64), ("iterations", C.c_int64),
                 ("graph_launches", C.c_int64), ("block_size", C.c_int), ("pool_paths", C.c_int),
                 ("lds_kib", C.c_int), ("refill", C.c_int), ("leaf_batch", C.c_int),
-                ("dir_bins", C.c_int), ("shade_block", C.c_int), ("shade_sort", C.c_int),
+                ("dir_bins", C.c_int), ("shade_block", C.c_int),
                 ("device_bytes", C.c_uint64),
                 ("stage_ms", C.c_double * 8), ("stage_items", C.c_int64 * 8), ("stage_launches", C.c_int64 * 8),
                 ("stage_bytes", C.c_int64 * 8), ("stage_threads", C.c_int64 * 8)]
@@ -212,10 +211,10 @@ def _iptr(a):
 
 def params(spp=64, seed=1234, max_depth=8, threads=0, stage_timing=False, queue_capacity=0, trace_bounces=False,
            block_size=0, pool_paths=0, scan_tile=0, lds_nodes=-1, lds_kib=0, refill=0, leaf_batch=0,
-           dir_bins=0, shade_block=0, graph_iters=0, shade_sort=0):
+           dir_bins=0, shade_block=0, graph_iters=0):
     return RenderParams(spp, seed, max_depth, threads, 0, int(stage_timing), queue_capacity, int(trace_bounces),
                         block_size, pool_paths, scan_tile, lds_nodes, lds_kib, refill, leaf_batch, dir_bins,
-                        shade_block, graph_iters, shade_sort)
+                        shade_block, graph_iters)
 
 
 class Scene:

@@ -36,8 +36,7 @@ sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 import wfamd  # noqa: E402
 
 # Persistent-engine tuning fields (wf_render_params) the autotuner chooses.
-TUNING_KEYS = ("block_size", "pool_paths", "lds_kib", "refill", "leaf_batch", "dir_bins", "shade_block",
-               "shade_sort")
+TUNING_KEYS = ("block_size", "pool_paths", "lds_kib", "refill", "leaf_batch", "dir_bins", "shade_block")
 
 # Pinned protocol (the reference pins its values in bench/protocol.json).
 PROTOCOL = {

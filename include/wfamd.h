@@ -59,8 +59,6 @@ typedef struct wf_render_params {
     int graph_iters;      /* persistent: pool iterations per captured graph launch (0 = max_depth + 2
                            * when every sample fits the pool, else 16; 8 in the serving loop;
                            * rounded up to even) */
-    int shade_sort;       /* persistent, one job: each shade round regrouped by material before
-                           * shading, 1 off, 2 on (0 = default) */
 } wf_render_params;
 
 /* RenderStats, include/wf/renderer.hpp:23-37. stage_ms: extend, classify, shade_lambert,
@@ -258,7 +256,7 @@ typedef struct wf_batch_stats {
     uint64_t extension_rays, shadow_rays;
     int64_t invalid_hits, queue_overflows;
     int64_t iterations, graph_launches;
-    int block_size, pool_paths, lds_kib, refill, leaf_batch, dir_bins, shade_block, shade_sort;
+    int block_size, pool_paths, lds_kib, refill, leaf_batch, dir_bins, shade_block;
     uint64_t device_bytes;
     /* With tuning->stage_timing: per-stage time, items, launches and algorithmic device bytes in
      * wf_render_stats' order (extend, -, shade, -, -, shadow, -, other); otherwise zero. */
