@@ -459,12 +459,13 @@ int wf_renderer_render(wf_renderer* r, const wf_render_params* p, wf_render_stat
         check_params(rp);
         wf::RenderStats st{};
         auto t0 = std::chrono::steady_clock::now();
-        wf::RenderResult res = r->r->render(rp, &st);
+        // GpuRenderer::render's steps, with the image finalized straight into the caller's buffers.
+        r->r->reset();
+        r->r->run_passes(0, rp.spp, rp, &st);
+        if (rgb || lum_var) r->r->download_to(rp.spp, rgb, lum_var);
         auto t1 = std::chrono::steady_clock::now();
         st.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         to_stats(st, stats);
-        if (rgb) std::memcpy(rgb, res.rgb.data(), res.rgb.size() * sizeof(float));
-        if (lum_var) std::memcpy(lum_var, res.lum_var.data(), res.lum_var.size() * sizeof(float));
         return (int)WF_OK;
     });
 }

@@ -454,6 +454,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=-1)
     ap.add_argument("--quick", action="store_true", help="small sizes (smoke run)")
     ap.add_argument("--out", default="")
+    ap.add_argument("--stage-profile", choices=["wavefront", "persistent"], default="",
+                    help="only one stage-timed render of the configuration's scene with this arm (configs 0-2): "
+                         "per stage ms, items, launches and algorithmic device-memory bytes")
+    ap.add_argument("--spp", type=int, default=0, help="samples per pixel of --stage-profile (0 = protocol)")
     args = ap.parse_args()
     cfg = dict(PROTOCOL["configs"][str(args.config)])
     if args.reps > 0:
@@ -461,6 +465,15 @@ def main():
     if args.warmup >= 0:
         cfg["warmup"] = args.warmup
     wfamd.lib()
+    if args.stage_profile:
+        if args.config > 2:
+            ap.error("--stage-profile takes configs 0-2")
+        spp = args.spp or cfg["spp"]
+        scene = wfamd.Scene(cfg["scene"], cfg["width"], cfg["height"])
+        st = stage_profile(scene, args.stage_profile, spp, PROTOCOL["seed"], PROTOCOL["max_depth"])
+        print(json.dumps({"config": args.config, "scene": cfg["scene"], "width": cfg["width"],
+                          "height": cfg["height"], "spp": spp, "arm": args.stage_profile, "stages": st}), flush=True)
+        return
     if args.config <= 2:
         doc = bench_scene(args.config, cfg, args.quick)
         summary = {"config": args.config, "verdict": doc["verdict"], "reasons": doc["reasons"],

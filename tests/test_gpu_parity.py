@@ -558,3 +558,26 @@ def test_full_size_sphere_grid_arms_agree(wf, scenes, oracle_ref):
     rgb, var = wf.finalize(*mk, spp)
     assert np.isfinite(rgb).all() and np.isfinite(var).all()
     assert mst.invalid_hits == 0 and pst.queue_overflows == 0
+
+
+def test_render_and_partial_downloads_at_720p(wf, scenes):
+    """wf_renderer_render finalizes straight into the caller's buffers, and wf_renderer_download
+    with one output pointer NULL gives the other output's bits (four-range pinned download)."""
+    import ctypes as C
+    s = scenes("sphere_grid", 1280, 720)
+    spp = 2
+    r = wf.Renderer(s, "persistent")
+    rgb, var, _ = r.render(wf.params(spp=spp))
+    want_rgb, want_var = wf.finalize(*r.download_sums(), spp)
+    assert_bitwise(rgb, want_rgb, "render rgb vs finalize of the sums")
+    assert_bitwise(var, want_var, "render var vs finalize of the sums")
+    full_rgb, full_var = r.download(spp)
+    only_rgb = np.full(3 * 1280 * 720, np.nan, np.float32)
+    only_var = np.full(1280 * 720, np.nan, np.float32)
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    wf._check(wf.lib().wf_renderer_download(r._h, spp, fp(only_rgb), None))
+    wf._check(wf.lib().wf_renderer_download(r._h, spp, None, fp(only_var)))
+    r.close()
+    assert_bitwise(only_rgb, full_rgb, "rgb-only download")
+    assert_bitwise(only_var, full_var, "lum_var-only download")
+    assert_bitwise(full_rgb, rgb, "download vs render")

@@ -1,0 +1,106 @@
+"""wfstats (graphics-book-tutorials_amd/wfstats.py), the analysis tool of tools/wfstats.cpp.
+
+CPU: the BVH section reproduces the reference's published tree statistics (results/stats/stats.json:
+node, leaf and depth counts and SAH cost), the RNG restatement equals the engine's own draws (the
+golden vectors written by the reference's code), and the statistics helpers give known answers.
+GPU (--quick): every section runs on real renders; the furnace arms agree with one another and
+with the geometric series, the error falls as spp^-1/2, survival counts never fall with the cap.
+"""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, PKG
+
+sys.path.insert(0, PKG)
+import wfstats  # noqa: E402
+
+PUBLISHED_BVH = {"cornell": (7, 4, 3, 7.47625002), "divergence": (563, 282, 10, 8.6194229),
+                 "trianglehv": (67279, 33640, 18, 29.0322913)}
+
+
+def test_bvh_section_reproduces_published_tree_statistics(wf):
+    out = wfstats.bvh(True)
+    for name, (nodes, leaves, depth, sah) in PUBLISHED_BVH.items():
+        s = out[name]
+        assert (s["nodes"], s["leaves"], s["max_depth"]) == (nodes, leaves, depth), name
+        assert s["sah_cost"] == pytest.approx(sah, rel=1e-6), name
+        assert s["internal"] + s["leaves"] == s["nodes"] and sum(s["leaf_depth_hist"]) == s["leaves"]
+        assert sum(k * c for k, c in enumerate(s["leaf_size_hist"], 1)) == s["primitives"]
+        assert all(0.0 < r <= 1.0 for r in s["child_to_parent_area_by_depth"])
+
+
+def test_rng_restatement_equals_the_engine_golden_draws():
+    g = json.load(open(os.path.join(GOLDEN, "rng.json")))
+    for x, want in g["hash_u32"].items():
+        assert int(wfstats._hash_u32(int(x))) == want
+    for seed, px, smp, k0, k1 in g["path_key"]:
+        a, b = wfstats.path_key(seed, px, smp)
+        assert (int(a), int(b)) == (k0, k1)
+    for k0, k1, depth, dim, bits in g["rng"]:
+        assert int(wfstats.rng(k0, k1, depth, dim).view(np.uint32)) == bits
+
+
+def test_rng_section_statistics():
+    r = wfstats.rng_stats(True)
+    assert abs(r["avalanche_mean"] - 0.5) < 5 * r["avalanche_se"] / np.sqrt(96 * 64)
+    assert r["avalanche_max_dev"] < 6 * r["avalanche_se"]
+    assert r["ks_d"] < r["ks_critical_5pct"] * 1.5
+    assert sum(abs(a) > r["acf_band95"] for a in r["acf"]) <= 10
+    chi = np.array(r["pair_chi2_by_seed"])
+    assert 0.8 * r["pair_dof"] < np.median(chi) < 1.2 * r["pair_dof"]
+
+
+def test_statistics_helpers_known_answers():
+    f = wfstats.ols([0, 1, 2, 3], [1, 3, 5, 7])
+    assert f["a"] == pytest.approx(1) and f["b"] == pytest.approx(2) and f["r2"] == pytest.approx(1)
+    assert f["se_b"] == pytest.approx(0, abs=1e-12)
+    flat = wfstats.lorenz_stats(np.ones(1000))
+    assert flat["gini"] == pytest.approx(0, abs=1e-9) and flat["top1pct_share"] == pytest.approx(0.01)
+    spike = np.zeros(1000)
+    spike[7] = 5.0
+    s = wfstats.lorenz_stats(spike)
+    assert s["gini"] == pytest.approx(0.99, abs=1e-9) and s["top1pct_share"] == 1.0 and s["max_over_mean"] == 1000
+    cat = np.zeros((4, 32), np.int64)
+    cat[:, 16:] = 1
+    cat[0, 0] = -1
+    t = wfstats.tile_stats(cat, 16, 2)
+    assert t["categories"] == [-1, 0, 1] and sum(t["probability"]) == pytest.approx(1)
+    assert t["mean_distinct_per_tile"] == pytest.approx((2 + 1 + 1 + 1) / 4)
+
+
+def test_unknown_section_is_an_error(tmp_path):
+    r = subprocess.run([sys.executable, os.path.join(PKG, "wfstats.py"), str(tmp_path / "o.json"), "nope"],
+                       capture_output=True, text=True)
+    assert r.returncode == 2 and "unknown section" in r.stderr
+
+
+@pytest.mark.gpu
+def test_every_section_on_the_gpu_quick(wf, tmp_path):
+    print("wfstats --quick, all sections", flush=True)
+    out = tmp_path / "stats.json"
+    r = subprocess.run([sys.executable, os.path.join(PKG, "wfstats.py"), str(out), "--quick"], capture_output=True,
+                       text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-3000:]
+    d = json.load(open(out))
+    assert set(wfstats.SECTIONS) <= set(d) and d["schema_version"] == 1 and d["device"] != "cpu"
+    fu = d["furnace"]
+    for k in ("megakernel", "wavefront", "persistent"):
+        assert fu[k] == fu["cpu"], k                 # every arm renders the same bits
+    for m, se, a0, a1 in zip(fu["cpu"], fu["cpu_se"], fu["series_to_depth_minus_1"], fu["series_to_depth"]):
+        assert min(a0, a1) - 5 * se <= m <= max(a0, a1) + 5 * se
+    cv = d["convergence"]
+    assert -0.8 < cv["loglog_fit"]["b"] < -0.3
+    for name, s in d["survival"].items():
+        e = s["extension_rays_by_cap"]
+        assert e[0] == s["paths"] and all(a <= b for a, b in zip(e, e[1:])), name
+    for name, s in d["divergence"].items():
+        assert sum(s["wave64"]["probability"]) == pytest.approx(1) and s["wave64"]["mean_distinct_per_tile"] >= 1
+    for name, s in d["scaling"].items():
+        assert all(t > 0 for t in s["megakernel_ms"] + s["wavefront_ms"] + s["persistent_ms"])
+    for name, s in d["variance"].items():
+        assert 0 <= s["gini"] < 1 and len(s["lorenz"]) == 101
