@@ -503,7 +503,7 @@ __device__ __forceinline__ void shade_load(const PCtx& c, const PathBuf& in, con
 // next extend see rays of nearby pixels heading the same way.
 template <bool SINGLE>
 __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, PathBuf out) {
-    __shared__ uint32_t cnt[2][10][16];   // per round parity: 8 direction bins, shadow, finished
+    __shared__ uint32_t cnt[2][9][16];    // per round parity, per wave: 8 direction bins, shadow
     __shared__ uint32_t bases[2][2];
     __shared__ uint32_t pre[kSegs + 1], bas[kSegs], win[kSegs];
     __shared__ JobTab jt;                   // multi-job only
@@ -591,29 +591,38 @@ __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, Path
             if (bin == b) ml = m;
             if (lane_id() == 0) cn[b][wave] = (uint32_t)__popcll(m);
         }
-        const uint64_t ms = __ballot(has != 0), mf = __ballot(fin);
-        if (lane_id() == 0) {
-            cn[8][wave] = (uint32_t)__popcll(ms);
-            cn[9][wave] = (uint32_t)__popcll(mf);
-        }
+        const uint64_t ms = __ballot(has != 0);
+        if (lane_id() == 0) cn[8][wave] = (uint32_t)__popcll(ms);
         __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t t0 = 0, t1 = 0, t2 = 0;
-            for (int b = 0; b < nb; ++b)
-                for (uint32_t w = 0; w < nwaves; ++w) {
-                    const uint32_t a = cn[b][w];
-                    cn[b][w] = t0;
-                    t0 += a;
+        // Wave 0 turns the counts into offsets: survivors bin-major then wave, shadow rays by wave
+        // (exclusive prefix sums over the lanes), and reserves the round's slots in its segment.
+        // A single job's finished paths are counted once per iteration (shadow kernel).
+        if (wave == 0) {
+            const uint32_t l = lane_id();
+            uint32_t* flat = &cn[0][0];   // [b][w] at b * 16 + w; lane l holds slots 2l, 2l + 1
+            const uint32_t s0 = 2u * l, s1 = s0 + 1u;
+            const bool v0 = (s0 >> 4) < (uint32_t)nb && (s0 & 15u) < nwaves;
+            const bool v1 = (s1 >> 4) < (uint32_t)nb && (s1 & 15u) < nwaves;
+            const uint32_t a0 = v0 ? flat[s0] : 0u, a1 = v1 ? flat[s1] : 0u;
+            const uint32_t c8 = l < nwaves ? cn[8][l] : 0u;
+            uint32_t x = a0 + a1, y = c8;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t xu = (uint32_t)__shfl_up((int)x, d), yu = (uint32_t)__shfl_up((int)y, d);
+                if ((int)l >= d) {
+                    x += xu;
+                    y += yu;
                 }
-            for (uint32_t w = 0; w < nwaves; ++w) {
-                const uint32_t b = cn[8][w], d = cn[9][w];
-                cn[8][w] = t1;
-                t1 += b;
-                t2 += d;
             }
-            bases[round][0] = win[seg] + (t0 ? atomicAdd(&st->seg_cnt[seg], t0) : 0u);
-            bases[round][1] = win[seg] + (t1 ? atomicAdd(&st->sh_cnt[seg], t1) : 0u);
-            (void)t2;   // a single job's finished paths are counted once per iteration (shadow kernel)
+            const uint32_t t0 = (uint32_t)__shfl((int)x, 63), t1 = (uint32_t)__shfl((int)y, 63);
+            const uint32_t e = x - a0 - a1;
+            if (v0) flat[s0] = e;
+            if (v1) flat[s1] = e + a0;
+            if (l < nwaves) cn[8][l] = y - c8;
+            uint32_t r = 0u;
+            if (l == 0 && t0) r = atomicAdd(&st->seg_cnt[seg], t0);
+            if (l == 1 && t1) r = atomicAdd(&st->sh_cnt[seg], t1);
+            if (l < 2) bases[round][l] = win[seg] + r;
         }
         __syncthreads();
         uint32_t o = 0;
