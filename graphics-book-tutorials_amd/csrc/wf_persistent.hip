@@ -288,7 +288,7 @@ __device__ __forceinline__ uint32_t seg_pos(const uint32_t* prefix, const uint32
     return base[s] + (i - prefix[s]);
 }
 
-// Slices of kFeedSlice items are dealt to the waves of a traversal kernel per workgroup: workgroup
+// Slices of `size` items are dealt to the waves of a traversal kernel per workgroup: workgroup
 // b owns the slices b, b + B, b + 2B, ... (B workgroups), which sample the whole item range, and its
 // waves take them in turn from a counter in LDS, the first one by wave index.
 // A wave that drew cheap rays so takes more of its workgroup's slices, and the workgroup ends when
@@ -296,22 +296,31 @@ __device__ __forceinline__ uint32_t seg_pos(const uint32_t* prefix, const uint32
 // once, by whichever wave. Wave-uniform state; the whole wave calls these.
 struct BlockFeed {
     uint32_t cursor = 0, end = 0;   // rest of the current slice [cursor, end)
+    uint32_t size = kFeedSlice;     // slice size of the launch
     bool dry = false;               // no slice left
 };
 
+// Slice sizes. A launch ends with its last wave, and a wave's last slice ends with its longest
+// ray, so shorter slices end a launch sooner while longer ones keep a wave on neighbouring rays
+// for longer. Extension rays keep 128 (64 made heightfield's extend slower, 256 both scenes' at
+// 2^24 slots); shadow launches are smaller and their rays cheaper, and take 64 (shadow time at
+// 2^22 slots: sphere_grid 3.74 -> 3.36 ms, heightfield 11.29 -> 10.94 ms).
+constexpr uint32_t kExtendSlice = kFeedSlice, kShadowSlice = 64;
+
 __device__ __forceinline__ void feed_take(BlockFeed& f, uint32_t n, uint32_t j) {
-    const unsigned long long c = ((unsigned long long)blockIdx.x + (unsigned long long)gridDim.x * j) * kFeedSlice;
+    const unsigned long long c = ((unsigned long long)blockIdx.x + (unsigned long long)gridDim.x * j) * f.size;
     if (c >= n) {
         f.dry = true;
         f.cursor = f.end = n;
     } else {
         f.cursor = (uint32_t)c;
-        f.end = min(f.cursor + kFeedSlice, n);
+        f.end = min(f.cursor + f.size, n);
     }
 }
 
 // `ctr` (LDS) must hold the workgroup's wave count before any wave calls feed_next.
-__device__ __forceinline__ void feed_first(BlockFeed& f, uint32_t n) {
+__device__ __forceinline__ void feed_first(BlockFeed& f, uint32_t n, uint32_t size) {
+    f.size = size;
     feed_take(f, n, (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)));
 }
 
@@ -322,8 +331,8 @@ __device__ __forceinline__ void feed_next(BlockFeed& f, uint32_t n, uint32_t* ct
 }
 
 // True when the workgroup owns no slice (its first is blockIdx.x): it returns before staging LDS.
-__device__ __forceinline__ bool feed_idle(uint32_t n) {
-    return (unsigned long long)blockIdx.x * kFeedSlice >= n;
+__device__ __forceinline__ bool feed_idle(uint32_t n, uint32_t size) {
+    return (unsigned long long)blockIdx.x * size >= n;
 }
 
 __global__ void pw_init_kernel(PState* st) {
@@ -406,7 +415,8 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
     const PState* st = c.st;
     const uint32_t n_act = st->it_act, n_live = st->it_live, slot0 = single_slot(st);
     const unsigned long long wbase = st->it_work_base;
-    if (feed_idle(n_act)) return;
+    const uint32_t fsize = kExtendSlice;
+    if (feed_idle(n_act, fsize)) return;
     __shared__ uint32_t pre[kSegs + 1], bas[kSegs], feed_ctr;
     if (threadIdx.x == 0) feed_ctr = blockDim.x >> 6;
     if (threadIdx.x <= kSegs) pre[threadIdx.x] = st->in_prefix[threadIdx.x];
@@ -415,7 +425,7 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
     else stage_job0(c, slot0, &job0);   // stage_lds synchronises
     stage_lds<SINGLE>(c, lds, &sp);   // synchronises
     BlockFeed f;
-    feed_first(f, n_act);
+    feed_first(f, n_act, fsize);
     LdsNodes src{};
     int nn = 0, sc = 0;
     if constexpr (SINGLE) {
@@ -685,11 +695,12 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
     }
     __syncthreads();
     const uint32_t n = pre[kSegs];
-    if (feed_idle(n)) return;
+    const uint32_t fsize = kShadowSlice;
+    if (feed_idle(n, fsize)) return;
     if constexpr (!SINGLE) stage_jobs(c, st, &jt, done_cnt, done_tag);
     stage_lds<SINGLE>(c, lds, &sp);
     BlockFeed f;
-    feed_first(f, n);
+    feed_first(f, n, fsize);
     LdsNodes src{};
     int nn = 0;
     if constexpr (SINGLE) {
