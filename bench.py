@@ -40,14 +40,6 @@ POOL_CAPS = (1 << 24, 1 << 22)
 POOL_STEPS = 10
 POOL_SLOT_BYTES = 184   # two 64 B path-state buffers, the 8 B hit and a 48 B shadow-queue entry per slot
 HF_STEPS = 10
-# Device-memory traffic against the algorithmic bytes, from rocprofv3 FETCH_SIZE and WRITE_SIZE on this
-# workload (sphere_grid 1280x720x64, default tuning), each counter scaled by what it reads on a plain
-# copy kernel of known size (FETCH_SIZE x2.0, WRITE_SIZE x1.0); source and commands:
-# profiles/persistent_sphere_grid_1280x720x64_counters.md. Those passes ran on an earlier version whose
-# path pool held 2^22 slots, so their bytes per launch do not describe the current launches (a
-# launch now covers up to 2^26 slots); only the measured / algorithmic ratio per kernel carries over, and
-# `traffic` is reported as null.
-EARLIER_TRAFFIC_RATIO = {"sphere_grid 1280x720x64": {"extend": 1.08, "shade": 1.03, "shadow": 3.06}}
 
 
 def percentile(v, p):
@@ -311,14 +303,12 @@ def main():
         launches = max(d["launches"], 1)
         avg_s = d["ms"] / 1e3 / launches
         achieved = d["bytes"] / launches / avg_s / 1e9
-        wl = f"{args.scene} {args.width}x{args.height}x{spp}"
-        ratio = EARLIER_TRAFFIC_RATIO.get(wl, {}).get(dom)
         roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                 "note": ("extend and shadow are BVH traversal: bound by the issue of their node visits and "
-                         "leaf tests (LDS latency, lane utilisation), not by HBM; traffic null: no counter "
-                         "pass since the pool grew to 2^26 slots"),
-                "traffic_over_algorithmic_earlier_version": ratio,
+                         "leaf tests (VALU issue, lane utilisation), not by HBM; traffic null: the persistent "
+                         "render is not run under rocprofv3 (a profiler crash in that path has no shown "
+                         "cause), so no FETCH_SIZE/WRITE_SIZE pass covers these kernels"),
                 "avg_launch_us": avg_s * 1e6, "bytes_per_launch": d["bytes"] / launches,
                 "items_per_launch": d["items"] / launches,
                 "stages": {k: {"ms": v["ms"], "launches": v["launches"], "items": v["items"],
