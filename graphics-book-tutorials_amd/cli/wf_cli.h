@@ -36,7 +36,7 @@ inline std::string usage() {
            "       --mode {cpu,megakernel,wavefront,persistent} --report PATH --out PREFIX --threads N --reps N "
            "--warmup N\n"
            "Scenes: cornell divergence trianglehv furnace cornell555 sphere_grid heightfield\n"
-           "Also:  wf validate-report REPORT.json | wf imgdiff A.pfm B.pfm [--tol T]\n";
+           "Also:  wf validate-report REPORT.json | wf imgdiff A.pfm B.pfm [--tol T] | wf gif OUT.gif [options]\n";
 }
 
 inline bool parse_int(const char* s, int& out) {

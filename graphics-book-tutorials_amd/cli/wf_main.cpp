@@ -11,6 +11,7 @@
 #include <memory>
 
 #include "wf_cli.h"
+#include "wf_gif.h"
 #include "wf_json.h"
 #include "wfamd.h"
 
@@ -102,9 +103,93 @@ int imgdiff_command(int argc, const char* argv[]) {
     return d.max_abs_rel <= tol ? 0 : 1;
 }
 
+// `wf gif OUT.gif [options]` (tools/wfgif.cpp; options in wf_gif.h): progressive-refinement or orbit
+// frames of GPU renders as one looping GIF. Exit 0 when written, 2 on a usage or render error.
+int gif_command(int argc, const char* argv[]) {
+    GifConfig c;
+    try {
+        c = parse_gif_args(argc, argv);
+    } catch (const ConfigError& e) {
+        std::cerr << "Error: " << e.what() << "\n" << gif_usage();
+        return 2;
+    }
+    const int W = c.width, H = c.height, scale = std::max(1, H / 126);
+    const int mode = mode_id(c.mode);
+    wf_render_params p{};
+    p.seed = c.seed;
+    p.max_depth = c.max_depth;
+    p.lds_nodes = -1;
+    GifWriter gif(c.out, W, H);
+    std::vector<float> mean((size_t)3 * W * H);
+    auto scene_upper = [](std::string s) {
+        for (char& ch : s) ch = (char)std::toupper((unsigned char)ch);
+        return s;
+    };
+    for (const std::string& name : c.scenes) {
+        wf_scene* scene = nullptr;
+        check(wf_scene_create(name.c_str(), W, H, &scene));
+        std::unique_ptr<wf_scene, void (*)(wf_scene*)> scene_guard(scene, wf_scene_destroy);
+        std::vector<std::vector<float>> frames;
+        std::vector<std::string> captions;
+        if (c.orbit == 0) {
+            wf_renderer* r = nullptr;
+            check(wf_renderer_create(scene, mode, &r));
+            std::unique_ptr<wf_renderer, void (*)(wf_renderer*)> r_guard(r, wf_renderer_destroy);
+            check(wf_renderer_reset(r));
+            int done = 0;
+            for (int target : c.steps) {
+                wf_render_stats st{};
+                p.spp = target - done;
+                check(wf_renderer_run_passes(r, done, target - done, &p, &st));
+                done = target;
+                check(wf_renderer_download_mean(r, done, mean.data()));
+                frames.push_back(mean);
+                captions.push_back(scene_upper(name) + "  " + std::to_string(done) + " SPP");
+            }
+        } else {
+            for (int k = 0; k < c.orbit; ++k) {
+                const float yaw = 360.0f * (float)k / (float)c.orbit;
+                wf_scene* view = nullptr;
+                check(wf_scene_view(scene, W, H, yaw, 1, &view));
+                std::unique_ptr<wf_scene, void (*)(wf_scene*)> view_guard(view, wf_scene_destroy);
+                wf_renderer* r = nullptr;
+                check(wf_renderer_create(view, mode, &r));
+                std::unique_ptr<wf_renderer, void (*)(wf_renderer*)> r_guard(r, wf_renderer_destroy);
+                wf_render_stats st{};
+                p.spp = c.spp;
+                check(wf_renderer_render(r, &p, &st, mean.data(), nullptr));
+                frames.push_back(mean);
+                captions.push_back(scene_upper(name) + "  " + std::to_string((int)std::lround(yaw)) + " DEG  " +
+                                   std::to_string(c.spp) + " SPP");
+            }
+        }
+        const float exposure = c.exposure > 0.0f ? c.exposure
+                                                 : gif_auto_exposure(c.orbit == 0 ? frames.back() : frames.front());
+        for (size_t k = 0; k < frames.size(); ++k) {
+            std::vector<uint8_t> rgb8 = tonemap_srgb8(frames[k], exposure);
+            const bool last = k + 1 == frames.size();
+            if (last && !c.png_prefix.empty()) write_png(c.png_prefix + name + ".png", W, H, rgb8);
+            gif_caption(rgb8, W, H, 3 * scale, 3 * scale, captions[k], scale);
+            gif.frame(gif_quantize(rgb8, W, H), last && c.orbit == 0 ? c.hold : c.delay);
+        }
+    }
+    const int n = gif.frames();
+    gif.close();
+    std::printf("wrote %s (%d frames, %dx%d)\n", c.out.c_str(), n, W, H);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, const char* argv[]) {
+    if (argc >= 2 && std::string(argv[1]) == "gif") {
+        try {
+            return gif_command(argc, argv);
+        } catch (const std::exception& e) {
+            std::cerr << "Error: " << e.what() << std::endl;
+            return 2;
+        }
+    }
     if (argc >= 2 && std::string(argv[1]) == "validate-report") return validate_report_command(argc, argv);
     if (argc >= 2 && std::string(argv[1]) == "imgdiff") {
         try {

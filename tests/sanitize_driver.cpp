@@ -13,6 +13,7 @@
 
 #include "wf_cli.h"
 #include "wf_cpu.h"
+#include "wf_gif.h"
 #include "wf_json.h"
 #include "wf_oracle.h"
 #include "wf_scene.h"
@@ -169,6 +170,30 @@ void cli_helpers(const std::string& dir) {
     expect(truncated, "truncated PFM refused");
     std::vector<double> v = {3, 1, 2};
     expect(percentile(v, 0.5) == 2.0, "percentile");
+    // GIF: quantise, caption (clipped at every edge), LZW over a table that fills and clears, file.
+    const int gw = 161, gh = 121;
+    std::vector<uint8_t> g8((size_t)3 * gw * gh);
+    uint32_t lcg = 1u;
+    for (size_t i = 0; i < g8.size(); ++i) {
+        lcg = lcg * 1664525u + 1013904223u;
+        g8[i] = (uint8_t)(lcg >> 24);
+    }
+    gif_caption(g8, gw, gh, -5, -3, "EDGE 0123456789 ABCDEFGHIJKLMNOPQRSTUVWXYZ .:-/%=+()", 2);
+    gif_caption(g8, gw, gh, gw - 4, gh - 2, "X", 3);
+    const std::vector<uint8_t> gi = gif_quantize(g8, gw, gh);
+    GifLzwEnd end;
+    (void)gif_lzw(gi.data(), gi.size(), &end);
+    expect(end.clears > 0, "gif lzw clears its table");
+    (void)gif_lzw(gi.data(), 1);
+    (void)gif_lzw(gi.data(), 0);
+    GifWriter gw_file(dir + "/s.gif", gw, gh);
+    gw_file.frame(gi, 7);
+    gw_file.frame(gif_quantize(std::vector<uint8_t>((size_t)3 * gw * gh, 200), gw, gh), 9);
+    gw_file.close();
+    expect(gif_auto_exposure(img) > 0.0f, "gif exposure");
+    const char* gargs[] = {"wf", "gif", "o.gif", "--steps", "1,3,9", "--scenes", "cornell,furnace", "--exposure", "0.5"};
+    const GifConfig gc = parse_gif_args(9, gargs);
+    expect(gc.steps.size() == 3 && gc.scenes.size() == 2 && gc.exposure == 0.5f, "parse_gif_args");
 }
 
 }  // namespace

@@ -1,0 +1,284 @@
+"""`wf gif`: the animation tool (tools/wfgif.cpp) and the viewer's orbit camera, headless.
+
+On the CPU (host helpers of graphics-book-tutorials_amd/cli/wf_gif.h through tests/cli_units.cpp):
+- LZW: every generated index image (uniform bytes that fill and clear the string table, runs,
+  a few symbols, a constant) decodes to the same indices with a strict decoder written here from
+  the GIF89a specification and with Pillow's decoder, including images whose end code must go out
+  one bit wider (the decoder's last table entry reaches 2^size exactly);
+- error diffusion: a flat colour comes out as palette colours whose mean is that colour, and a
+  colour on the palette comes out as that one entry;
+- the file: frame count, size, delays, the looping extension, a caption drawn in white;
+- option parsing: defaults, every option, each rejected value.
+On the GPU: `wf gif` renders the same progressive frames with every GPU arm, so the files are
+byte-identical (the arms agree bit for bit); the orbit animation has one distinct frame per angle.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import PKG, ROOT
+
+CLI_DIR = os.path.join(PKG, "cli")
+WF_BIN = os.path.join(PKG, "lib", "wf")
+
+
+@pytest.fixture(scope="module")
+def units(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("gif") / "cli_units")
+    subprocess.run(["g++", "-std=c++20", "-O1", "-Wall", "-I", CLI_DIR, os.path.join(ROOT, "tests", "cli_units.cpp"),
+                    "-o", exe], check=True)
+
+    def run(*args):
+        out = subprocess.run([exe, *map(str, args)], check=True, capture_output=True, text=True).stdout
+        return json.loads(out)
+    return run
+
+
+# ---------------------------------------------------------------------------------------------
+# an independent reader: GIF89a blocks and variable-width LZW (the specification's appendix F)
+# ---------------------------------------------------------------------------------------------
+def lzw_decode(data, min_size):
+    clear, end = 1 << min_size, (1 << min_size) + 1
+    pos = acc = nbits = 0
+    size = min_size + 1
+    table, prev, out = None, None, bytearray()
+
+    def read(width):
+        nonlocal pos, acc, nbits
+        while nbits < width:
+            if pos >= len(data):
+                raise ValueError("code stream ends without an end code")
+            acc |= data[pos] << nbits
+            pos += 1
+            nbits += 8
+        c = acc & ((1 << width) - 1)
+        acc >>= width
+        nbits -= width
+        return c
+
+    while True:
+        c = read(size)
+        if c == clear:
+            table = [bytes([i]) for i in range(clear)] + [b"", b""]
+            size, prev = min_size + 1, None
+            continue
+        if table is None:
+            raise ValueError("data before the first clear code")
+        if c == end:
+            break
+        if prev is None:
+            if c >= clear:
+                raise ValueError(f"first code {c} after a clear is not a literal")
+            out += table[c]
+            prev = c
+            continue
+        if c < len(table):
+            entry = table[c]
+        elif c == len(table):
+            entry = table[prev] + table[prev][:1]
+        else:
+            raise ValueError(f"code {c} beyond the table ({len(table)} entries)")
+        if len(table) < 4096:
+            table.append(table[prev] + entry[:1])
+        out += entry
+        prev = c
+        if len(table) == (1 << size) and size < 12:
+            size += 1
+    return bytes(out)
+
+
+def read_gif(path):
+    b = open(path, "rb").read()
+    assert b[:6] == b"GIF89a"
+    w, h = b[6] | b[7] << 8, b[8] | b[9] << 8
+    assert b[10] & 0x80 and (b[10] & 7) == 7, "a global colour table of 256 entries"
+    palette = np.frombuffer(b[13:13 + 768], np.uint8).reshape(256, 3)
+    p = 13 + 768
+    frames, delays, loops = [], [], False
+    while True:
+        t = b[p]
+        if t == 0x3B:
+            break
+        if t == 0x21:
+            label = b[p + 1]
+            p += 2
+            body = bytearray()
+            while b[p]:
+                body += b[p + 1:p + 1 + b[p]]
+                p += 1 + b[p]
+            p += 1
+            if label == 0xF9:
+                delays.append(body[1] | body[2] << 8)
+            if label == 0xFF and bytes(body[:11]) == b"NETSCAPE2.0":
+                loops = True
+            continue
+        assert t == 0x2C, f"unexpected block {t:#x}"
+        fx, fy, fw, fh = (b[p + 1 + 2 * k] | b[p + 2 + 2 * k] << 8 for k in range(4))
+        assert (fx, fy, fw, fh) == (0, 0, w, h) and b[p + 9] == 0
+        p += 10
+        min_size = b[p]
+        p += 1
+        data = bytearray()
+        while b[p]:
+            data += b[p + 1:p + 1 + b[p]]
+            p += 1 + b[p]
+        p += 1
+        frames.append(np.frombuffer(lzw_decode(bytes(data), min_size), np.uint8))
+    return w, h, palette, frames, delays, loops
+
+
+def pillow_frames(path):
+    """Pillow's decode of every frame, as RGB (Pillow may hand frames after the first back in RGB)."""
+    Image = pytest.importorskip("PIL.Image")
+    im = Image.open(path)
+    out = []
+    for k in range(im.n_frames):
+        im.seek(k)
+        out.append(np.array(im.convert("RGB")).reshape(-1, 3))
+    return im.size, out
+
+
+@pytest.mark.parametrize("w,h,kind,seed", [
+    (300, 200, "random", 1),      # the table fills and clears many times
+    (257, 131, "runs", 3),        # long strings, 12-bit codes
+    (257, 131, "few", 3),
+    (64, 48, "const", 0),
+    (1, 1, "random", 5),
+    (2, 1, "random", 6),
+    (41, 44, "random", 239),      # end code one bit wider (the decoder's table reaches 2048)
+    (41, 196, "random", 391),     # the same at 512, after two clears
+])
+def test_lzw_round_trip(units, tmp_path, w, h, kind, seed):
+    path = str(tmp_path / "f.gif")
+    end = units("gifidx", path, w, h, kind, seed)
+    want = np.fromfile(path + ".idx", np.uint8)
+    gw, gh, _, frames, _, _ = read_gif(path)
+    assert (gw, gh) == (w, h) and len(frames) == 1
+    assert np.array_equal(frames[0], want)
+    size, pf = pillow_frames(path)
+    _, _, pal, _, _, _ = read_gif(path)
+    assert size == (w, h) and np.array_equal(pf[0], pal[want])
+    if seed in (239, 391):
+        assert end["wide_end"], end
+    if kind == "random" and w * h >= 60000:
+        assert end["clears"] > 0
+
+
+def test_palette_is_332_rgb(units, tmp_path):
+    path = str(tmp_path / "p.gif")
+    units("gifidx", path, 4, 4, "const", 0)
+    _, _, pal, _, _, _ = read_gif(path)
+    r = np.round(np.arange(8) * 255 / 7)
+    b = np.round(np.arange(4) * 255 / 3)
+    i = np.arange(256)
+    assert np.array_equal(pal[:, 0], r[i >> 5]) and np.array_equal(pal[:, 1], r[(i >> 2) & 7])
+    assert np.array_equal(pal[:, 2], b[i & 3])
+
+
+@pytest.mark.parametrize("rgb", [(100, 150, 200), (17, 240, 5), (200, 13, 128), (128, 128, 128)])
+def test_error_diffusion_keeps_a_flat_colour_on_average(units, rgb):
+    q = units("gifquant", 96, 96, *rgb)
+    assert np.allclose(q["mean"], rgb, atol=1.5), q
+    assert q["distinct"] > 1
+
+
+@pytest.mark.parametrize("rgb", [(0, 0, 0), (255, 255, 255), (73, 146, 170), (219, 36, 85)])
+def test_a_palette_colour_is_reproduced_exactly(units, rgb):
+    q = units("gifquant", 33, 17, *rgb)
+    assert q == {"mean": list(map(float, rgb)), "distinct": 1}
+
+
+def test_animation_file(units, tmp_path):
+    path = str(tmp_path / "a.gif")
+    assert units("gifanim", path, 40, 30) == {"frames": 3}
+    w, h, pal, frames, delays, loops = read_gif(path)
+    assert (w, h, len(frames), delays, loops) == (40, 30, 3, [10, 20, 30], True)
+    first = np.fromfile(path + ".idx", np.uint8)
+    assert np.array_equal(frames[0], first)
+    img = frames[0].reshape(h, w)
+    # "ABC 123" at (2, 2), scale 1: the dots of 'A' (row 0 col 1, row 2 cols 0-2) are white (255),
+    # the box under the caption (one dot of margin) is dark.
+    assert img[2, 3] == 255 and all(img[4, 2 + k] == 255 for k in range(3))
+    assert tuple(pal[img[2, 2]]) != (255, 255, 255) and pal[img[2, 2]].max() <= 109
+    assert np.array_equal(frames[1], np.full(w * h, frames[1][0])) or len(set(frames[1])) <= 4
+    size, pf = pillow_frames(path)
+    assert size == (40, 30) and len(pf) == 3 and all(np.array_equal(a, pal[b]) for a, b in zip(pf, frames))
+
+
+# ---------------------------------------------------------------------------------------------
+# options
+# ---------------------------------------------------------------------------------------------
+def test_gif_option_defaults(units):
+    c = units("gifparse", "out.gif")
+    assert c == {"out": "out.gif", "scenes": ["cornell", "divergence", "trianglehv"], "width": 448, "height": 252,
+                 "mode": "megakernel", "steps": [1, 2, 4, 8, 16, 32, 64, 128, 256], "orbit": 0, "spp": 16,
+                 "seed": 1234, "max_depth": 8, "exposure": 0, "delay": 25, "hold": 150, "png_prefix": ""}
+
+
+def test_gif_every_option(units):
+    c = units("gifparse", "x.gif", "--scenes", "cornell555,heightfield", "--width", "64", "--height", "36",
+              "--mode", "persistent", "--steps", "1,5,9", "--orbit", "12", "--spp", "3", "--seed", "7",
+              "--max-depth", "4", "--exposure", "2.5", "--delay", "8", "--hold", "90", "--final-png-prefix", "pre_")
+    assert c == {"out": "x.gif", "scenes": ["cornell555", "heightfield"], "width": 64, "height": 36,
+                 "mode": "persistent", "steps": [1, 5, 9], "orbit": 12, "spp": 3, "seed": 7, "max_depth": 4,
+                 "exposure": 2.5, "delay": 8, "hold": 90, "png_prefix": "pre_"}
+
+
+@pytest.mark.parametrize("args,flag", [
+    (["--width", "0"], "--width"), (["--height", "4097"], "--height"), (["--mode", "cpu"], "--mode"),
+    (["--steps", "2,1"], "--steps"), (["--steps", "0"], "--steps"), (["--steps", "1,x"], "--steps"),
+    (["--orbit", "-1"], "--orbit"), (["--spp", "0"], "--spp"), (["--max-depth", "65"], "--max-depth"),
+    (["--exposure", "0"], "--exposure"), (["--exposure", "nan"], "--exposure"), (["--delay", "0"], "--delay"),
+    (["--scenes", "a,,b"], "--scenes"), (["--bogus", "1"], "unknown flag"), (["--spp"], "--spp"),
+])
+def test_gif_rejects(units, args, flag):
+    c = units("gifparse", "o.gif", *args)
+    assert "error" in c and flag in c["error"], c
+
+
+def test_wf_gif_usage_errors_exit_2(tmp_path):
+    assert os.path.exists(WF_BIN), "build the CLI first (make -C graphics-book-tutorials_amd)"
+    for args in (["gif"], ["gif", str(tmp_path / "o.gif"), "--steps", "4,2"]):
+        r = subprocess.run([WF_BIN, *args], capture_output=True, text=True)
+        assert r.returncode == 2 and "Usage: wf gif" in r.stderr, (args, r.stderr)
+
+
+# ---------------------------------------------------------------------------------------------
+# GPU
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.gpu
+def test_wf_gif_progressive_frames_are_the_same_with_every_gpu_arm(tmp_path):
+    files = {}
+    for mode in ("megakernel", "wavefront", "persistent"):
+        out = tmp_path / f"{mode}.gif"
+        print(f"[gif] progressive cornell,sphere_grid 64x36 steps 1,2,4 mode {mode}", flush=True)
+        r = subprocess.run([WF_BIN, "gif", str(out), "--scenes", "cornell,sphere_grid", "--width", "64", "--height",
+                            "36", "--steps", "1,2,4", "--mode", mode, "--final-png-prefix", str(tmp_path / f"{mode}_")],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert "6 frames, 64x36" in r.stdout
+        files[mode] = out.read_bytes()
+        assert (tmp_path / f"{mode}_cornell.png").exists() and (tmp_path / f"{mode}_sphere_grid.png").exists()
+    assert files["megakernel"] == files["wavefront"] == files["persistent"]
+    w, h, _, frames, delays, loops = read_gif(str(tmp_path / "megakernel.gif"))
+    assert (w, h, len(frames), loops) == (64, 36, 6, True)
+    assert delays == [25, 25, 150, 25, 25, 150]
+    assert not np.array_equal(frames[0], frames[2]) and not np.array_equal(frames[2], frames[5])
+
+
+@pytest.mark.gpu
+def test_wf_gif_orbit_has_one_frame_per_angle(tmp_path):
+    out = tmp_path / "orbit.gif"
+    print("[gif] orbit sphere_grid 48x27, 4 frames of 2 spp, persistent", flush=True)
+    r = subprocess.run([WF_BIN, "gif", str(out), "--scenes", "sphere_grid", "--width", "48", "--height", "27",
+                        "--orbit", "4", "--spp", "2", "--mode", "persistent", "--exposure", "1.5"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    w, h, _, frames, delays, _ = read_gif(str(out))
+    assert (w, h, len(frames), delays) == (48, 27, 4, [25, 25, 25, 25])
+    for i in range(4):
+        for j in range(i + 1, 4):
+            assert not np.array_equal(frames[i], frames[j]), (i, j)
