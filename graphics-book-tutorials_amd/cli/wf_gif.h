@@ -21,17 +21,45 @@
 
 namespace wfcli {
 
-// ---- palette: "3-3-2" RGB, 8 red x 8 green x 4 blue levels, index = r << 5 | g << 2 | b ----
-constexpr int kGifLevels[3] = {8, 8, 4};
+// ---- palette: a 6 x 6 x 6 colour cube (entries 0..215, index r * 36 + g * 6 + b, levels 0, 51,
+// ..., 255) and a ramp of 40 greys (entries 216..255, 255 (k + 1) / 41 rounded), so that the
+// neutral skies and walls of the scenes dither between greys rather than between colours ----
+constexpr int kGifCube = 6, kGifGreys = 40;
 
-inline int gif_level_value(int c, int q) { return (int)std::lround(255.0 * q / (kGifLevels[c] - 1)); }
+inline int gif_cube_value(int q) { return 51 * q; }
+inline int gif_grey_value(int k) { return (int)std::lround(255.0 * (k + 1) / (kGifGreys + 1)); }
 
 inline void gif_palette(uint8_t* table768) {
-    for (int i = 0; i < 256; ++i) {
-        table768[3 * i + 0] = (uint8_t)gif_level_value(0, (i >> 5) & 7);
-        table768[3 * i + 1] = (uint8_t)gif_level_value(1, (i >> 2) & 7);
-        table768[3 * i + 2] = (uint8_t)gif_level_value(2, i & 3);
+    for (int i = 0; i < 216; ++i) {
+        table768[3 * i + 0] = (uint8_t)gif_cube_value(i / 36);
+        table768[3 * i + 1] = (uint8_t)gif_cube_value(i / 6 % 6);
+        table768[3 * i + 2] = (uint8_t)gif_cube_value(i % 6);
     }
+    for (int k = 0; k < kGifGreys; ++k)
+        for (int c = 0; c < 3; ++c) table768[3 * (216 + k) + c] = (uint8_t)gif_grey_value(k);
+}
+
+// Nearest palette entry to v (squared distance): the cube's nearest is per channel; the ramp's is
+// the grey nearest to the channels' mean. The cube wins ties.
+inline int gif_nearest(const float* v, float* chosen) {
+    float cube[3], dc = 0.0f;
+    int q[3];
+    for (int c = 0; c < 3; ++c) {
+        q[c] = std::clamp((int)std::lround(v[c] / 51.0f), 0, kGifCube - 1);
+        cube[c] = (float)gif_cube_value(q[c]);
+        dc += (v[c] - cube[c]) * (v[c] - cube[c]);
+    }
+    const float mean = (v[0] + v[1] + v[2]) / 3.0f;
+    const int k = std::clamp((int)std::lround(mean * (kGifGreys + 1) / 255.0f) - 1, 0, kGifGreys - 1);
+    const float g = (float)gif_grey_value(k);
+    float dg = 0.0f;
+    for (int c = 0; c < 3; ++c) dg += (v[c] - g) * (v[c] - g);
+    if (dg < dc) {
+        chosen[0] = chosen[1] = chosen[2] = g;
+        return 216 + k;
+    }
+    for (int c = 0; c < 3; ++c) chosen[c] = cube[c];
+    return q[0] * 36 + q[1] * 6 + q[2];
 }
 
 // Floyd-Steinberg error diffusion (Floyd and Steinberg, 1976) onto the palette, left to right and
@@ -44,17 +72,17 @@ inline std::vector<uint8_t> gif_quantize(const std::vector<uint8_t>& rgb8, int w
     for (int y = 0; y < h; ++y) {
         std::fill(nxt.begin(), nxt.end(), 0.0f);
         for (int x = 0; x < w; ++x) {
-            int q[3];
+            float v[3], got[3];
+            for (int c = 0; c < 3; ++c)
+                v[c] = std::clamp((float)rgb8[3 * ((size_t)y * w + x) + c] + cur[3 * (x + 1) + c], 0.0f, 255.0f);
+            idx[(size_t)y * w + x] = (uint8_t)gif_nearest(v, got);
             for (int c = 0; c < 3; ++c) {
-                const float v = std::clamp((float)rgb8[3 * ((size_t)y * w + x) + c] + cur[3 * (x + 1) + c], 0.0f, 255.0f);
-                q[c] = (int)std::lround(v * (kGifLevels[c] - 1) / 255.0f);
-                const float e = v - (float)gif_level_value(c, q[c]);
+                const float e = v[c] - got[c];
                 cur[3 * (x + 2) + c] += e * (7.0f / 16.0f);
                 nxt[3 * (x + 0) + c] += e * (3.0f / 16.0f);
                 nxt[3 * (x + 1) + c] += e * (5.0f / 16.0f);
                 nxt[3 * (x + 2) + c] += e * (1.0f / 16.0f);
             }
-            idx[(size_t)y * w + x] = (uint8_t)(q[0] << 5 | q[1] << 2 | q[2]);
         }
         std::swap(cur, nxt);
     }

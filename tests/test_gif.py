@@ -167,25 +167,32 @@ def test_lzw_round_trip(units, tmp_path, w, h, kind, seed):
         assert end["clears"] > 0
 
 
-def test_palette_is_332_rgb(units, tmp_path):
+def test_palette_is_a_colour_cube_and_a_grey_ramp(units, tmp_path):
     path = str(tmp_path / "p.gif")
     units("gifidx", path, 4, 4, "const", 0)
     _, _, pal, _, _, _ = read_gif(path)
-    r = np.round(np.arange(8) * 255 / 7)
-    b = np.round(np.arange(4) * 255 / 3)
-    i = np.arange(256)
-    assert np.array_equal(pal[:, 0], r[i >> 5]) and np.array_equal(pal[:, 1], r[(i >> 2) & 7])
-    assert np.array_equal(pal[:, 2], b[i & 3])
+    i = np.arange(216)
+    assert np.array_equal(pal[:216], np.stack([51 * (i // 36), 51 * (i // 6 % 6), 51 * (i % 6)], 1))
+    k = np.arange(40)
+    grey = np.floor(255 * (k + 1) / 41 + 0.5)
+    assert np.array_equal(pal[216:], np.stack([grey] * 3, 1))
 
 
-@pytest.mark.parametrize("rgb", [(100, 150, 200), (17, 240, 5), (200, 13, 128), (128, 128, 128)])
+@pytest.mark.parametrize("rgb", [(100, 150, 200), (17, 240, 5), (200, 13, 128), (128, 128, 128), (250, 250, 250)])
 def test_error_diffusion_keeps_a_flat_colour_on_average(units, rgb):
     q = units("gifquant", 96, 96, *rgb)
     assert np.allclose(q["mean"], rgb, atol=1.5), q
     assert q["distinct"] > 1
 
 
-@pytest.mark.parametrize("rgb", [(0, 0, 0), (255, 255, 255), (73, 146, 170), (219, 36, 85)])
+def test_a_grey_dithers_between_greys(units):
+    # (128, 128, 128) lies between ramp greys 124 and 131: only greys are used, no colour noise
+    q = units("gifquant", 64, 64, 128, 128, 128)
+    assert q["distinct"] == 2 and q["mean"][0] == q["mean"][1] == q["mean"][2]
+
+
+@pytest.mark.parametrize("rgb", [(0, 0, 0), (255, 255, 255), (51, 153, 204), (255, 0, 102), (137, 137, 137),
+                                 (6, 6, 6)])
 def test_a_palette_colour_is_reproduced_exactly(units, rgb):
     q = units("gifquant", 33, 17, *rgb)
     assert q == {"mean": list(map(float, rgb)), "distinct": 1}
@@ -199,9 +206,10 @@ def test_animation_file(units, tmp_path):
     first = np.fromfile(path + ".idx", np.uint8)
     assert np.array_equal(frames[0], first)
     img = frames[0].reshape(h, w)
-    # "ABC 123" at (2, 2), scale 1: the dots of 'A' (row 0 col 1, row 2 cols 0-2) are white (255),
-    # the box under the caption (one dot of margin) is dark.
-    assert img[2, 3] == 255 and all(img[4, 2 + k] == 255 for k in range(3))
+    # "ABC 123" at (2, 2), scale 1: the dots of 'A' (row 0 col 1, row 2 cols 0-2) are white (the
+    # cube's white or the ramp's lightest grey, as the diffused error falls), the box under the
+    # caption (one dot of margin) is dark.
+    assert all(pal[img[y, x]].min() >= 249 for y, x in [(2, 3)] + [(4, 2 + k) for k in range(3)])
     assert tuple(pal[img[2, 2]]) != (255, 255, 255) and pal[img[2, 2]].max() <= 109
     assert np.array_equal(frames[1], np.full(w * h, frames[1][0])) or len(set(frames[1])) <= 4
     size, pf = pillow_frames(path)
