@@ -61,7 +61,7 @@ struct PersistentJob {
 };
 
 struct PersistentTuning {
-    int block_size = 0;    // threads per workgroup: 256, 512 or 1024 (0 -> 512)
+    int block_size = 0;    // threads per workgroup: 256, 512 or 1024 (0 -> 1024)
     int pool_paths = 0;    // path slots = queue capacity (0 -> the render's work, up to 2^26)
     int lds_kib = 0;       // LDS per workgroup for BVH nodes and primitive records (0 -> 64 KiB; at most the
                            // device limit less the kernels' own LDS, under 1 KiB)

@@ -47,7 +47,7 @@ typedef struct wf_render_params {
     int stage_timing;     /* wavefront: per-stage event timing (synchronising) */
     int queue_capacity;   /* wavefront test hook: 0 = path count; smaller -> counted overflows */
     int trace_bounces;    /* wavefront: record per-bounce integer work, see wf_renderer_trace */
-    int block_size;       /* persistent: threads per workgroup (256, 512, 1024) */
+    int block_size;       /* persistent: threads per workgroup (256, 512, 1024; 0 = 1024) */
     int pool_paths;       /* persistent: path slots (= queue capacity) */
     int scan_tile;        /* plain wavefront: compaction tile (elements) */
     int lds_nodes;        /* megakernel / plain wavefront: BVH nodes staged in LDS, -1 = default */
