@@ -61,10 +61,10 @@ struct PersistentJob {
 };
 
 struct PersistentTuning {
-    int block_size = 0;    // threads per workgroup: 256, 512 or 1024 (0 -> 1024)
+    int block_size = 0;    // threads per workgroup: 256, 512 or 1024 (0 -> 1024 for one job, 512 for several)
     int pool_paths = 0;    // path slots = queue capacity (0 -> the render's work, up to 2^26)
-    int lds_kib = 0;       // LDS per workgroup for BVH nodes and primitive records (0 -> 64 KiB; at most the
-                           // device limit less the kernels' own LDS, under 1 KiB)
+    int lds_kib = 0;       // LDS per workgroup for BVH nodes and primitive records (0 -> 64 KiB for one job,
+                           // 40 KiB for several; at most the device limit less the kernels' own LDS)
     int graph_iters = 0;   // iterations per captured graph (0 -> max_depth + 2 when every sample fits
                            // the pool, else 16; rounded up to even)
     int refill = 0;        // traversal refill threshold in lanes of 64 (0 -> 32)

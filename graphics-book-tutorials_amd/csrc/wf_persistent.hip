@@ -22,13 +22,16 @@ namespace {
 // runs in depth + 1 iterations (sphere_grid 1280x720x64: 41 iterations at 2^22 slots, 19.1 ms;
 // 9 at 2^26, 14.4 ms).
 constexpr int kDefaultPool = 1 << 26;
-// 1024: a workgroup of 16 waves and its LDS tree top fill half a CU at 8 waves per SIMD; with 512
-// threads a 64 KiB tree top allows only two 8-wave workgroups per CU (4 waves per SIMD):
-// heightfield 31.4 -> 22.7 ms per 1280x720x64 render, sphere_grid 14.0 -> 13.4 ms, serving p99
-// 124 -> 107 ms.
-constexpr int kDefaultBlock = 1024;
+// Default workgroup size and LDS budget of the traversal kernels, by mode. A single job's kernels
+// (60-64 VGPRs) run 8 waves per SIMD: two 1024-thread workgroups with a 64 KiB tree top each fill
+// a CU, where 512-thread workgroups with the same top fit only two per CU (4 waves per SIMD;
+// heightfield 31.4 -> 22.7 ms per 1280x720x64 render with 1024). The multi-job kernels (72
+// VGPRs) have room for 28 waves per CU: one 1024-thread workgroup uses 16, three 512-thread ones
+// with 40 KiB tops 24 (the 64-job batch 38.4 -> 34.3 ms, serving at 60 requests/s p99 119 -> 90
+// ms). A tuning's block size or LDS budget, when given, applies to both modes.
+constexpr int kDefaultBlock = 1024, kDefaultBlockMulti = 512;
 constexpr int kDefaultGraphIters = 16;
-constexpr int kDefaultLdsKiB = 64;
+constexpr int kDefaultLdsKiB = 64, kDefaultLdsKiBMulti = 40;
 constexpr int kDefaultRefill = 32;
 constexpr int kDefaultLeafBatch = 16;
 constexpr int kDefaultDirBins = 1;
@@ -809,6 +812,7 @@ struct PersistentEngine::Impl {
     hipStream_t stream = nullptr;
     DevCounters* ctr = nullptr;
     int block = kDefaultBlock, graph_iters = kDefaultGraphIters, lds_bytes = kDefaultLdsKiB * 1024, cus = 0;
+    int block_multi = kDefaultBlockMulti, lds_multi = kDefaultLdsKiBMulti * 1024;   // multi-job mode
     int lds_max = 0;      // LDS a workgroup may use on this device (bytes)
     int lds_static = 0;   // allowance for the traversal kernels' own LDS (bytes)
     int refill = kDefaultRefill;
@@ -916,6 +920,7 @@ struct PersistentEngine::LaunchPlan {
     PCtx c;
     bool single;
     size_t lds_bytes;
+    int block;   // traversal workgroup size of the mode
     int gx, gs, gw;
 };
 
@@ -937,6 +942,7 @@ void PersistentEngine::ensure_pool(const PersistentTuning& t) {
     int cap = t.pool_paths > 0 ? t.pool_paths : kDefaultPool;
     cap = (cap + 1023) & ~1023;
     m.block = t.block_size > 0 ? t.block_size : kDefaultBlock;
+    m.block_multi = t.block_size > 0 ? t.block_size : kDefaultBlockMulti;
     if (m.block != 256 && m.block != 512 && m.block != 1024)
         throw std::runtime_error("persistent: block size must be 256, 512 or 1024");
     m.refill = t.refill > 0 ? t.refill : kDefaultRefill;
@@ -954,12 +960,14 @@ void PersistentEngine::ensure_pool(const PersistentTuning& t) {
     if (t.graph_iters > 1024) throw std::runtime_error("persistent: graph iterations must be 1..1024");
     m.graph_iters = t.graph_iters > 0 ? ((t.graph_iters + 1) & ~1) : kDefaultGraphIters;
     const int kib = t.lds_kib > 0 ? t.lds_kib : kDefaultLdsKiB;
+    const int kib_multi = t.lds_kib > 0 ? t.lds_kib : kDefaultLdsKiBMulti;
     // The budget is the dynamic LDS; the kernels' own (the plan and the segment tables)
     // comes on top.
     if (kib * 1024 > m.lds_max - m.lds_static)
         throw std::runtime_error("persistent: LDS budget above what a workgroup may use on this device");
-    if (m.lds_bytes != kib * 1024) {
-        m.lds_bytes = kib * 1024;
+    m.lds_bytes = kib * 1024;   // single mode plans its scene's LDS per launch plan
+    if (m.lds_multi != kib_multi * 1024) {
+        m.lds_multi = kib_multi * 1024;
         if (!m.hscenes.empty()) {
             update_lds_plan();
             m.scene_gen += 1;
@@ -1094,7 +1102,7 @@ void PersistentEngine::update_lds_plan() {
         nn[k] = m.hscenes[k].n_nodes;
         np[k] = m.hscenes[k].n_prims;
     }
-    m.lds_plan = plan_lds(nn, np, n, m.lds_bytes);
+    m.lds_plan = plan_lds(nn, np, n, m.lds_multi);
     pw_put_kernel<LdsPlan><<<1, 1, 0, m.stream>>>(m.ctx.plan, m.lds_plan);
     check(hipGetLastError(), "put lds plan");
 }
@@ -1261,9 +1269,10 @@ PersistentEngine::LaunchPlan PersistentEngine::make_plan(bool single) {
         for (const void* fn : {fx, fw})
             check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.lds_bytes),
                   "hipFuncSetAttribute");
-    L.gx = grid_for(fx, m.block, L.lds_bytes, m.cus);
+    L.block = single ? m.block : m.block_multi;
+    L.gx = grid_for(fx, L.block, L.lds_bytes, m.cus);
     L.gs = grid_for(fs, m.shade_block, 0, m.cus);
-    L.gw = grid_for(fw, m.block, L.lds_bytes, m.cus);
+    L.gw = grid_for(fw, L.block, L.lds_bytes, m.cus);
     return L;
 }
 
@@ -1279,14 +1288,14 @@ void PersistentEngine::launch_iteration(const LaunchPlan& L, int par, hipEvent_t
     if (ev) check(hipEventRecord(ev[0], st), "hipEventRecord");
     pw_plan_kernel<<<1, 1, 0, st>>>(m.st, c.jobs, m.ctr, c.cap, (uint32_t)m.shade_block);
     if (ev) check(hipEventRecord(ev[1], st), "hipEventRecord");
-    if (L.single) pw_extend_kernel<true><<<L.gx, m.block, L.lds_bytes, st>>>(c, in);
-    else pw_extend_kernel<false><<<L.gx, m.block, L.lds_bytes, st>>>(c, in);
+    if (L.single) pw_extend_kernel<true><<<L.gx, L.block, L.lds_bytes, st>>>(c, in);
+    else pw_extend_kernel<false><<<L.gx, L.block, L.lds_bytes, st>>>(c, in);
     if (ev) check(hipEventRecord(ev[2], st), "hipEventRecord");
     if (L.single) pw_shade_kernel<true><<<L.gs, m.shade_block, 0, st>>>(c, in, out);
     else pw_shade_kernel<false><<<L.gs, m.shade_block, 0, st>>>(c, in, out);
     if (ev) check(hipEventRecord(ev[3], st), "hipEventRecord");
-    if (L.single) pw_shadow_kernel<true><<<L.gw, m.block, L.lds_bytes, st>>>(c, out);
-    else pw_shadow_kernel<false><<<L.gw, m.block, L.lds_bytes, st>>>(c, out);
+    if (L.single) pw_shadow_kernel<true><<<L.gw, L.block, L.lds_bytes, st>>>(c, out);
+    else pw_shadow_kernel<false><<<L.gw, L.block, L.lds_bytes, st>>>(c, out);
     if (ev) check(hipEventRecord(ev[4], st), "hipEventRecord");
 }
 
@@ -1308,7 +1317,7 @@ void PersistentEngine::run(int launches) {
     k.single = m.seq_hi - m.seq_lo == 1;
     k.single_scene = k.single ? m.ring[(m.seq_hi - 1) & kJobMask].scene : -1;
     k.scene_gen = m.scene_gen;
-    k.block = m.block;
+    k.block = k.single ? m.block : m.block_multi;
     k.refill = m.refill;
     k.leaf_batch = m.leaf_batch;
     k.dir_bins = m.dir_bins;
@@ -1363,9 +1372,9 @@ void PersistentEngine::run_profiled(StageTimes* out) {
         out->launches[0] += 1;
         out->launches[2] += 1;
         out->launches[5] += 1;
-        out->threads[0] = (long long)L.gx * m.block;
+        out->threads[0] = (long long)L.gx * L.block;
         out->threads[2] = (long long)L.gs * m.shade_block;
-        out->threads[5] = (long long)L.gw * m.block;
+        out->threads[5] = (long long)L.gw * L.block;
         out->launches[7] += 1;
     }
     // The captured graph runs an even number of iterations from buffer 0; the work has drained,
@@ -1647,14 +1656,20 @@ PersistentTuning autotune_persistent(const PersistentJob* jobs, int n_jobs, hipS
         t.dir_bins = nb;
         time(t);
     }
-    // Stage 3: LDS budget.
+    // Stage 3: LDS budget, jointly with the workgroup size: the two decide together how many
+    // workgroups fit a CU (a 128 KiB tree top admits one, 48 KiB three 512-thread ones), and the
+    // multi-job kernels' 72 VGPRs leave room for 28 waves per CU, so a 512-thread workgroup with a
+    // smaller tree top can beat the workgroup size stage 1 found at 64 KiB (the 64-job batch:
+    // 512 threads with 48 KiB, 35.6 ms, against 1024 with 128 KiB, 38.4 ms).
     base = best;
-    for (int kib : {32, 40, 48, 64, 128}) {
-        if (kib == base.lds_kib) continue;
-        PersistentTuning t = base;
-        t.lds_kib = kib;
-        time(t);
-    }
+    for (int bs : {512, 1024})
+        for (int kib : {32, 40, 48, 64, 128}) {
+            if (bs == base.block_size && kib == base.lds_kib) continue;
+            PersistentTuning t = base;
+            t.block_size = bs;
+            t.lds_kib = kib;
+            time(t);
+        }
     // Stage 4: a probe's iterations hold less work than the whole batch's, and one probe render
     // is one sample of a noisy time, so the three fastest distinct candidates and the default
     // tuning are timed again on the whole batch (fastest of two renders each), and the fastest
@@ -1672,10 +1687,8 @@ PersistentTuning autotune_persistent(const PersistentJob* jobs, int n_jobs, hipS
         if (!dup) finalists.push_back(c.first);
         if (finalists.size() == 3) break;
     }
-    PersistentTuning dflt;
+    PersistentTuning dflt;   // block size and LDS budget 0: the engine's defaults for the mode
     dflt.pool_paths = base_pool;
-    dflt.block_size = kDefaultBlock;
-    dflt.lds_kib = kDefaultLdsKiB;
     dflt.refill = kDefaultRefill;
     dflt.leaf_batch = kDefaultLeafBatch;
     dflt.dir_bins = kDefaultDirBins;

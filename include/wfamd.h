@@ -47,11 +47,11 @@ typedef struct wf_render_params {
     int stage_timing;     /* wavefront: per-stage event timing (synchronising) */
     int queue_capacity;   /* wavefront test hook: 0 = path count; smaller -> counted overflows */
     int trace_bounces;    /* wavefront: record per-bounce integer work, see wf_renderer_trace */
-    int block_size;       /* persistent: threads per workgroup (256, 512, 1024; 0 = 1024) */
+    int block_size;       /* persistent: threads per workgroup (256, 512, 1024; 0 = 1024 for one job, 512 for several) */
     int pool_paths;       /* persistent: path slots (= queue capacity) */
     int scan_tile;        /* plain wavefront: compaction tile (elements) */
     int lds_nodes;        /* megakernel / plain wavefront: BVH nodes staged in LDS, -1 = default */
-    int lds_kib;          /* persistent: LDS per workgroup for BVH nodes + primitive records (KiB) */
+    int lds_kib;          /* persistent: LDS per workgroup for BVH nodes + primitive records (KiB; 0 = 64 for one job, 40 for several) */
     int refill;           /* persistent: a wave refetches rays when fewer lanes than this traverse */
     int leaf_batch;       /* persistent: parked leaves that start a wave's leaf-test round */
     int dir_bins;         /* persistent: survivor grouping, 1 none or 8 direction octants */
