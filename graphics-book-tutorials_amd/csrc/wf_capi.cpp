@@ -835,6 +835,16 @@ extern "C" int wf_batch_render(const wf_job* jobs, int n_jobs, const wf_render_p
     });
 }
 
+extern "C" uint64_t wf_cached_memory_bytes(void) { return wf::gpu::cached_block_bytes(); }
+
+extern "C" int wf_release_cached_memory(uint64_t* freed_bytes) {
+    return guard([&] {
+        const uint64_t n = wf::gpu::release_cached_blocks();
+        if (freed_bytes) *freed_bytes = n;
+        return (int)WF_OK;
+    });
+}
+
 extern "C" int wf_autotune(const wf_job* jobs, int n_jobs, int probe_spp, wf_render_params* tuning) {
     return wf_autotune_budget(jobs, n_jobs, probe_spp, 0, tuning);
 }

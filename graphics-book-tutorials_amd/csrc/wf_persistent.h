@@ -64,7 +64,7 @@ struct PersistentTuning {
     int block_size = 0;    // threads per workgroup: 256, 512 or 1024 (0 -> 1024 for one job, 512 for several)
     int pool_paths = 0;    // path slots = queue capacity (0 -> the render's work, up to 2^26)
     int lds_kib = 0;       // LDS per workgroup for BVH nodes and primitive records (0 -> 64 KiB for one job,
-                           // 40 KiB for several; at most the device limit less the kernels' own LDS)
+                           // 32 KiB for several; at most the device limit less the kernels' own LDS)
     int graph_iters = 0;   // iterations per captured graph (0 -> max_depth + 2 when every sample fits
                            // the pool, else 16; rounded up to even)
     int refill = 0;        // traversal refill threshold in lanes of 64 (0 -> 32)
@@ -231,5 +231,10 @@ LdsPlan plan_lds(const int* n_nodes, const int* n_prims, int n, long long budget
 PersistentTuning autotune_persistent(const PersistentJob* jobs, int n_jobs, hipStream_t stream, int probe_spp,
                                      std::vector<std::pair<PersistentTuning, double>>* trials,
                                      uint64_t state_budget = 0);
+
+// Large device blocks of ended engines, kept for the next engine of the same block size: the bytes
+// kept now, and freeing them all (returns the bytes freed).
+uint64_t cached_block_bytes();
+uint64_t release_cached_blocks();
 
 }  // namespace wf::gpu

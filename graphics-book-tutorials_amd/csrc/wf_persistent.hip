@@ -9,6 +9,7 @@
 // before the next classify, so every path computes the megakernel's bits.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "wf_persistent.h"
 
@@ -25,13 +26,14 @@ constexpr int kDefaultPool = 1 << 26;
 // Default workgroup size and LDS budget of the traversal kernels, by mode. A single job's kernels
 // (60-64 VGPRs) run 8 waves per SIMD: two 1024-thread workgroups with a 64 KiB tree top each fill
 // a CU, where 512-thread workgroups with the same top fit only two per CU (4 waves per SIMD;
-// heightfield 31.4 -> 22.7 ms per 1280x720x64 render with 1024). The multi-job kernels (72
-// VGPRs) have room for 28 waves per CU: one 1024-thread workgroup uses 16, three 512-thread ones
-// with 40 KiB tops 24 (the 64-job batch 38.4 -> 34.3 ms, serving at 60 requests/s p99 119 -> 90
-// ms). A tuning's block size or LDS budget, when given, applies to both modes.
+// heightfield 31.4 -> 22.7 ms per 1280x720x64 render with 1024). The multi-job extend (64 VGPRs)
+// and shadow (69) kernels run four and three 512-thread workgroups per CU with 32 KiB tops, where
+// one 1024-thread workgroup would use 16 of the 32 wave slots (the 64-job batch 38.4 -> 32.9 ms,
+// serving at 60 requests/s p99 119 -> 89 ms). A tuning's block size or LDS budget, when given,
+// applies to both modes.
 constexpr int kDefaultBlock = 1024, kDefaultBlockMulti = 512;
 constexpr int kDefaultGraphIters = 16;
-constexpr int kDefaultLdsKiB = 64, kDefaultLdsKiBMulti = 40;
+constexpr int kDefaultLdsKiB = 64, kDefaultLdsKiBMulti = 32;
 constexpr int kDefaultRefill = 32;
 constexpr int kDefaultLeafBatch = 16;
 constexpr int kDefaultDirBins = 1;
@@ -245,6 +247,66 @@ __device__ __forceinline__ LdsNodes src_of(const SceneView& s, const Node* lds, 
                         sp->prim_cnt[sc] > 0 ? lprim + sp->prim_off[sc] : nullptr);
 }
 
+// Scene source of the multi-job traversal kernels, the same nodes and primitive tests as LdsNodes
+// (src_of) in fewer registers: a lane keeps its scene slot and the place and size of its tree top
+// in LDS; the scene's global node and primitive pointers and its LDS primitive records are read
+// from the workgroup's LDS tables where they are needed (a node below the top, a leaf), which is
+// where a visit waits on memory anyway. Three VGPRs per lane instead of nine.
+struct MultiSrc {
+    const JobTab* jt;           // LDS (wave-uniform)
+    const LdsPlan* sp;          // LDS (wave-uniform)
+    lds_float4* prim_base;      // LDS primitive records of every scene (wave-uniform)
+    lds_float4* top;            // this lane's tree top
+    int n_lds, sc;
+    __device__ __forceinline__ void set(const Node* lds, int slot) {
+        sc = slot;
+        top = (lds_float4*)(lds + sp->node_off[slot]);
+        n_lds = sp->node_cnt[slot];
+    }
+    __device__ __forceinline__ Node node_lds(int n) const {
+        const f32x4 a = top[2 * n], b = top[2 * n + 1];
+        Node out;
+        out.lo = Vec3{a.x, a.y, a.z};
+        out.skip = __float_as_int(a.w);
+        out.hi = Vec3{b.x, b.y, b.z};
+        out.child = __float_as_int(b.w);
+        return out;
+    }
+    __device__ __forceinline__ Node node(int n) const {
+        Node out;
+        if (n < n_lds) {
+            out = node_lds(n);
+            __builtin_amdgcn_sched_barrier(0);
+        } else {
+            out = load_node_global(jt->scenes[sc].nodes, n);
+        }
+        return out;
+    }
+    __device__ __forceinline__ bool all_lds(int n_nodes) const { return n_lds >= n_nodes; }
+    __device__ __forceinline__ bool hit(int slot, const Ray& r, float* t) const {
+        const SceneView& s = jt->scenes[sc];
+        const glb_float4* g = (const glb_float4*)s.prims + 3 * slot;
+        if (sp->prim_cnt[sc] > 0) {
+            const f32x4 q = prim_base[sp->prim_off[sc] + slot];
+            if (q.w >= 0.0f) return hit_sphere(Vec3{q.x, q.y, q.z}, q.w, r, t);
+            const f32x4 b = g[1], c = g[2];
+            return hit_triangle(Vec3{q.x, q.y, q.z}, Vec3{b.x, b.y, b.z}, Vec3{c.x, c.y, c.z}, r, t);
+        }
+        const f32x4 a = g[0], b = g[1];
+        if (__float_as_int(a.w) < s.n_spheres) return hit_sphere(Vec3{a.x, a.y, a.z}, b.w, r, t);
+        const f32x4 c = g[2];
+        return hit_triangle(Vec3{a.x, a.y, a.z}, Vec3{b.x, b.y, b.z}, Vec3{c.x, c.y, c.z}, r, t);
+    }
+};
+
+__device__ __forceinline__ MultiSrc multi_src(const JobTab* jt, const LdsPlan* sp, const Node* lds) {
+    MultiSrc m{};
+    m.jt = jt;
+    m.sp = sp;
+    m.prim_base = (lds_float4*)(lds + sp->total_nodes);
+    return m;
+}
+
 // Adds one per lane with pred to the finished-path count of job slot jid, one update per distinct
 // slot in the wave, into the workgroup's LDS counters: a direct-mapped cache of kLdsJobs entries
 // whose tag[e] names the slot entry e counts for (the first slot with e = slot mod kLdsJobs to
@@ -434,6 +496,7 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
     BlockFeed f;
     feed_first(f, n_act, fsize);
     LdsNodes src{};
+    MultiSrc msrc = multi_src(&jt, &sp, lds);
     int nn = 0, sc = 0;
     if constexpr (SINGLE) {
         src = src_of(c.scene0, lds, &sp, 0);
@@ -466,7 +529,7 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
                     if constexpr (!SINGLE) sc = (int)scn;
                 }
                 if constexpr (!SINGLE) {
-                    src = src_of(jt.scenes[sc], lds, &sp, sc);
+                    msrc.set(lds, sc);
                     nn = jt.scenes[sc].n_nodes;
                 }
                 tr = traversal_begin(r);
@@ -478,7 +541,9 @@ __global__ __launch_bounds__(1024) void pw_extend_kernel(PCtx c, PathBuf in) {
             if (f.dry) break;
             continue;   // slice used up: take the next one
         }
-        wave_traverse(nn, src, tr, leaf, false, (f.dry && f.cursor >= f.end) ? 1 : c.refill, c.leaf_batch);
+        const int limit = (f.dry && f.cursor >= f.end) ? 1 : c.refill;
+        if constexpr (SINGLE) wave_traverse(nn, src, tr, leaf, false, limit, c.leaf_batch);
+        else wave_traverse(nn, msrc, tr, leaf, false, limit, c.leaf_batch);
         if (item != kIdle && tr.node < 0 && !leaf_parked(leaf)) {
             Hit h{tr.r.tmax, tr.prim};
             const SceneView* hs;
@@ -709,6 +774,7 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
     BlockFeed f;
     feed_first(f, n, fsize);
     LdsNodes src{};
+    MultiSrc msrc = multi_src(&jt, &sp, lds);
     int nn = 0;
     if constexpr (SINGLE) {
         src = src_of(c.scene0, lds, &sp, 0);
@@ -738,7 +804,7 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
                 r.tmax = o.w;
                 if constexpr (!SINGLE) {
                     const int sc = (int)((tag >> kTagSceneShift) & 7u);
-                    src = src_of(jt.scenes[sc], lds, &sp, sc);
+                    msrc.set(lds, sc);
                     nn = jt.scenes[sc].n_nodes;
                 }
                 tr = traversal_begin(r);
@@ -750,7 +816,9 @@ __global__ __launch_bounds__(1024) void pw_shadow_kernel(PCtx c, PathBuf out) {
             if (f.dry) break;
             continue;   // slice used up: take the next one
         }
-        wave_traverse(nn, src, tr, leaf, true, (f.dry && f.cursor >= f.end) ? 1 : c.refill, c.leaf_batch);
+        const int limit = (f.dry && f.cursor >= f.end) ? 1 : c.refill;
+        if constexpr (SINGLE) wave_traverse(nn, src, tr, leaf, true, limit, c.leaf_batch);
+        else wave_traverse(nn, msrc, tr, leaf, true, limit, c.leaf_batch);
         bool fin = false, clear = false;
         uint32_t jid = 0;
         if (item != kIdle && tr.node < 0 && !leaf_parked(leaf)) {
@@ -800,6 +868,92 @@ uint64_t job_pixels(const PersistentJob& j) {
     return j.has_view ? (uint64_t)j.width * (uint64_t)j.height : (uint64_t)j.scene->width * j.scene->height;
 }
 
+// Large device blocks (path state, sample arenas) of engines that have ended, kept for the next
+// engine that asks for the same size on the same device. A work-sized pool is up to 12 GB in a
+// dozen blocks; freeing and allocating that much for every batch render made the device stall
+// now and then for 0.1-2 s (a batch render's teardown went 15 -> 335 ms, its render 34 ms -> 2.2 s,
+// after about 16 renders in one process). Blocks below kMinBytes go straight back to the runtime;
+// at most kMaxHeld bytes are kept, per process; wf_release_cached_memory returns them.
+class BlockCache {
+public:
+    static constexpr size_t kMinBytes = 64ull << 20, kMaxHeld = 32ull << 30;
+
+    static void* get(size_t bytes, const char* what) {
+        if (bytes >= kMinBytes) {
+            int dev = 0;
+            check(hipGetDevice(&dev), "hipGetDevice");
+            std::lock_guard<std::mutex> lock(mu());
+            std::vector<Entry>& v = entries();
+            for (size_t i = 0; i < v.size(); ++i)
+                if (v[i].dev == dev && v[i].bytes == bytes) {
+                    void* p = v[i].p;
+                    held() -= bytes;
+                    v.erase(v.begin() + (std::ptrdiff_t)i);
+                    return p;
+                }
+        }
+        void* q = nullptr;
+        check(hipMalloc(&q, bytes), what);
+        return q;
+    }
+
+    // No kernel may still use p (the caller has synchronised the device).
+    static void put(void* p, size_t bytes) {
+        if (!p) return;
+        int dev = 0;
+        if (bytes >= kMinBytes && hipGetDevice(&dev) == hipSuccess) {
+            std::lock_guard<std::mutex> lock(mu());
+            if (held() + bytes <= kMaxHeld) {
+                entries().push_back(Entry{dev, bytes, p});
+                held() += bytes;
+                return;
+            }
+        }
+        (void)hipFree(p);
+    }
+
+    // Frees every kept block (of every device); returns the bytes freed.
+    static uint64_t release() {
+        std::lock_guard<std::mutex> lock(mu());
+        uint64_t n = 0;
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        for (const Entry& e : entries()) {
+            (void)hipSetDevice(e.dev);
+            (void)hipFree(e.p);
+            n += e.bytes;
+        }
+        (void)hipSetDevice(cur);
+        entries().clear();
+        held() = 0;
+        return n;
+    }
+
+    static uint64_t held_bytes() {
+        std::lock_guard<std::mutex> lock(mu());
+        return held();
+    }
+
+private:
+    struct Entry {
+        int dev;
+        size_t bytes;
+        void* p;
+    };
+    static std::mutex& mu() {
+        static std::mutex m;
+        return m;
+    }
+    static std::vector<Entry>& entries() {
+        static std::vector<Entry> v;
+        return v;
+    }
+    static size_t& held() {
+        static size_t h = 0;
+        return h;
+    }
+};
+
 int grid_for(const void* fn, int block, size_t lds, int cus) {
     int per = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, block, lds) != hipSuccess || per < 1) per = 1;
@@ -819,7 +973,7 @@ struct PersistentEngine::Impl {
     int leaf_batch = kDefaultLeafBatch;
     int dir_bins = kDefaultDirBins;
     int shade_block = kDefaultShadeBlock;
-    std::vector<void*> pool_allocs;
+    std::vector<std::pair<void*, size_t>> pool_allocs;   // blocks of the path pool and their sizes
     PState* st = nullptr;
     PState* st_host = nullptr;       // pinned mirror
     uint32_t* done_host = nullptr;   // pinned mirror of job_done
@@ -887,13 +1041,18 @@ struct PersistentEngine::Impl {
         exec = nullptr;
         graph = nullptr;
     }
+    // Blocks go to BlockCache once no kernel can still use them.
     void free_arena() {
-        if (arena) (void)hipFree(arena);
+        if (arena) {
+            (void)hipDeviceSynchronize();
+            BlockCache::put(arena, (size_t)arena_bytes);
+        }
         arena = nullptr;
         arena_bytes = 0;
     }
     void free_pool() {
-        for (void* p : pool_allocs) (void)hipFree(p);
+        if (!pool_allocs.empty()) (void)hipDeviceSynchronize();
+        for (const auto& [p, bytes] : pool_allocs) BlockCache::put(p, bytes);
         pool_allocs.clear();
     }
     ~Impl() {
@@ -984,9 +1143,8 @@ void PersistentEngine::ensure_pool(const PersistentTuning& t) {
     pool_bytes_ = 0;
     cap_ = cap;
     auto alloc = [&](auto** p, size_t bytes) {
-        void* q = nullptr;
-        check(hipMalloc(&q, bytes), "hipMalloc persistent pool");
-        m.pool_allocs.push_back(q);
+        void* q = BlockCache::get(bytes, "hipMalloc persistent pool");
+        m.pool_allocs.push_back({q, bytes});
         *p = (std::remove_reference_t<decltype(**p)>*)q;
         pool_bytes_ += bytes;
     };
@@ -1233,8 +1391,11 @@ void PersistentEngine::admit_chunk(const std::vector<PersistentJob>& chunk) {
     for (const PersistentJob& j : chunk) need += job_pixels(j) * (uint64_t)j.spp * sizeof(float4);
     if (need > m.arena_bytes) {
         m.free_arena();
-        check(hipMalloc(&m.arena, (size_t)need), "hipMalloc sample arena");
-        m.arena_bytes = need;
+        // Whole 64 MiB units, so that batches of similar sizes reuse each other's cached arena.
+        const uint64_t unit = 64ull << 20;
+        const uint64_t bytes = (need + unit - 1) / unit * unit;
+        m.arena = (float4*)BlockCache::get((size_t)bytes, "hipMalloc sample arena");
+        m.arena_bytes = bytes;
     }
     uint64_t off = 0;
     for (const PersistentJob& j : chunk) {
@@ -1723,5 +1884,8 @@ PersistentTuning autotune_persistent(const PersistentJob* jobs, int n_jobs, hipS
     (void)hipFree(ctr);
     return best;
 }
+
+uint64_t cached_block_bytes() { return BlockCache::held_bytes(); }
+uint64_t release_cached_blocks() { return BlockCache::release(); }
 
 }  // namespace wf::gpu

@@ -128,6 +128,7 @@ EXPORTS = [
     "wf_batch_render", "wf_autotune", "wf_autotune_budget", "wf_server_create", "wf_server_destroy", "wf_server_submit",
     "wf_server_step", "wf_server_poll", "wf_server_stats_get", "wf_test_sincos", "wf_test_sincos_grid_hash",
     "wf_test_intersect", "wf_test_slab", "wf_test_divu32", "wf_test_lds_plan", "wf_test_trace_paths", "wf_copy_probe",
+    "wf_cached_memory_bytes", "wf_release_cached_memory",
 ]
 
 
@@ -184,6 +185,7 @@ def lib():
         "wf_test_lds_plan": (I, [C.POINTER(C.c_int), C.POINTER(C.c_int), I, C.c_int64, C.POINTER(C.c_int),
                                  C.POINTER(C.c_int)]),
         "wf_test_trace_paths": (I, [P, C.POINTER(C.c_int), I, C.c_uint32, I, F, C.POINTER(C.c_uint32), I]),
+        "wf_cached_memory_bytes": (U64, []), "wf_release_cached_memory": (I, [C.POINTER(C.c_uint64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -397,6 +399,18 @@ def query_device():
     return dict(name=f.name.decode(), arch=f.arch.decode(), compute_units=f.compute_units,
                 runtime_version=f.runtime_version, driver_version=f.driver_version,
                 total_memory_bytes=f.total_memory_bytes, free_memory_bytes=f.free_memory_bytes)
+
+
+def cached_memory_bytes():
+    """Device bytes of ended persistent engines kept for reuse (wf_cached_memory_bytes)."""
+    return int(lib().wf_cached_memory_bytes())
+
+
+def release_cached_memory():
+    """Frees the kept blocks; returns the bytes freed (wf_release_cached_memory)."""
+    n = C.c_uint64()
+    _check(lib().wf_release_cached_memory(C.byref(n)))
+    return n.value
 
 
 def copy_probe(nbytes, reps=10):

@@ -51,7 +51,7 @@ typedef struct wf_render_params {
     int pool_paths;       /* persistent: path slots (= queue capacity) */
     int scan_tile;        /* plain wavefront: compaction tile (elements) */
     int lds_nodes;        /* megakernel / plain wavefront: BVH nodes staged in LDS, -1 = default */
-    int lds_kib;          /* persistent: LDS per workgroup for BVH nodes + primitive records (KiB; 0 = 64 for one job, 40 for several) */
+    int lds_kib;          /* persistent: LDS per workgroup for BVH nodes + primitive records (KiB; 0 = 64 for one job, 32 for several) */
     int refill;           /* persistent: a wave refetches rays when fewer lanes than this traverse */
     int leaf_batch;       /* persistent: parked leaves that start a wave's leaf-test round */
     int dir_bins;         /* persistent: survivor grouping, 1 none or 8 direction octants */
@@ -228,6 +228,16 @@ int wf_test_lds_plan(const int* n_nodes, const int* n_prims, int n, int64_t budg
  * counts (extension, shadow). */
 int wf_test_trace_paths(const wf_scene* scene, const int* pxys, int n, uint32_t seed, int max_depth, float* L,
                         uint32_t* rays, int device);
+
+/* ---- memory (no reference counterpart) ---- */
+/* Persistent engines (renderers, batches, the serving loop) hand their large device blocks (path
+ * state and sample arenas, 64 MiB and up) to a process-wide cache when they end, and the next
+ * engine that needs a block of the same size takes it from there: allocating and freeing a 12 GB
+ * work-sized pool for every batch render stalled the device for up to 2 s now and then. At most
+ * 32 GiB is kept. wf_cached_memory_bytes: what the cache holds; wf_release_cached_memory frees it
+ * all (no engine may be running; freed_bytes may be NULL). */
+uint64_t wf_cached_memory_bytes(void);
+int wf_release_cached_memory(uint64_t* freed_bytes);
 
 /* ---- measurement (no reference counterpart) ---- */
 /* Runs a plain 16-byte-per-lane device copy of `bytes` (multiple of 16) reps times and returns the

@@ -9,6 +9,9 @@
   created in its place render their own geometry.
 - Create/destroy cycles of renderers, batches and servers leave no device memory behind
   (tests/catch/wavefront_failure.cu:136-166, tests/test_gpu_queues.cu:193).
+- The large blocks of an ended engine are kept for the next engine of the same size: a second
+  renderer takes them without new device memory and renders the same bits; releasing the cache
+  gives the memory back.
 """
 import ctypes as C
 import gc
@@ -171,6 +174,35 @@ def free_bytes(wf):
     from hipmem import sync
     sync()
     return wf.query_device()["free_memory_bytes"]
+
+
+def test_large_blocks_of_ended_engines_are_reused_and_released(wf):
+    print("[cache] sphere_grid 1280x720x16 persistent, twice", flush=True)
+    wf.release_cached_memory()
+    assert wf.cached_memory_bytes() == 0
+    s = wf.Scene("sphere_grid", 1280, 720)
+    p = wf.params(spp=16)
+    base = free_bytes(wf)
+    r = wf.Renderer(s, "persistent")
+    rgb1, var1, _ = r.render(p)
+    r.close()
+    kept = wf.cached_memory_bytes()
+    # the work-sized pool: 14,745,600 slots, its ten 236 MB arrays and two 118 MB ones
+    assert kept >= 2_000_000_000, kept
+    assert base - free_bytes(wf) >= kept - (64 << 20)
+    before = free_bytes(wf)
+    r = wf.Renderer(s, "persistent")
+    rgb2, var2, _ = r.render(p)
+    during = free_bytes(wf)
+    r.close()
+    print(f"[cache] kept {kept}, device memory taken by the second renderer {before - during}", flush=True)
+    assert before - during < (256 << 20)          # its pool came from the cache
+    assert_bitwise(rgb2, rgb1, "second render rgb")
+    assert_bitwise(var2, var1, "second render var")
+    assert wf.cached_memory_bytes() == kept
+    freed = wf.release_cached_memory()
+    assert freed == kept and wf.cached_memory_bytes() == 0
+    assert free_bytes(wf) - before >= kept - (64 << 20)
 
 
 def test_create_destroy_cycles_leave_no_device_memory(wf):
