@@ -940,13 +940,14 @@ private:
         size_t bytes;
         void* p;
     };
+    // Never destroyed: an engine may end during process exit, after static destructors have run.
     static std::mutex& mu() {
-        static std::mutex m;
-        return m;
+        static std::mutex* m = new std::mutex;
+        return *m;
     }
     static std::vector<Entry>& entries() {
-        static std::vector<Entry> v;
-        return v;
+        static std::vector<Entry>* v = new std::vector<Entry>;
+        return *v;
     }
     static size_t& held() {
         static size_t h = 0;
