@@ -602,17 +602,26 @@ WF_HD PathState init_path(const SceneView& s, int px, int py, uint32_t sample, u
     return init_path_view(s.camera, s.width, s.height, px, py, sample, seed);
 }
 
+// The geometric normal is v / l: v = p - center and l = radius for a sphere, v = e1 x e2 and
+// l = |v| for a triangle (normalize: v itself when |v| = 0). One division serves both kinds, so a
+// wave whose lanes hit spheres and triangles divides once.
 WF_HD Surface make_surface(const SceneView& s, const Ray& ray, const Hit& h) {
     Surface f;
     const Prim& pr = s.prims[h.prim];
     f.p = ray.o + ray.d * h.t;
-    if (prim_is_sphere(s, pr)) {
-        f.n_out = (f.p - pr.p0) / pr.radius;
+    const bool sph = prim_is_sphere(s, pr);
+    Vec3 v;
+    float l;
+    if (sph) {
+        v = f.p - pr.p0;
+        l = pr.radius;
     } else {
         Vec3 e1 = pr.p1 - pr.p0;
         Vec3 e2 = pr.p2 - pr.p0;
-        f.n_out = normalize(cross(e1, e2));
+        v = cross(e1, e2);
+        l = length(v);
     }
+    f.n_out = (sph || l > 0.0f) ? v / l : v;
     f.material = pr.material;
     f.inside = dot(f.n_out, ray.d) > 0.0f ? 1 : 0;
     f.n = f.inside ? -f.n_out : f.n_out;
@@ -725,39 +734,44 @@ WF_HD void set_next_ray(PathState& ps, Vec3 origin, Vec3 dir) {
 // Lambertian (include/wf/path_core.hpp:192-228): next-event estimation (shadow ray and its
 // contribution, added by the caller if unoccluded, before the next classify), cosine-sampled
 // scatter, roulette. Returns true if the path continues.
+// Next-event estimation of a Lambertian hit from `org`: the shadow ray and its contribution
+// (shade_lambert's first half). Sets *has_shadow to 0 when there is none.
+WF_HD void lambert_nee(const SceneView& s, const PathState& ps, const Surface& f, const Material& m, Vec3 org,
+                       Ray* shadow, Vec3* contrib, int* has_shadow) {
+    *has_shadow = 0;
+    if (s.n_sel <= 0) return;
+    LightSample ls = sample_light(s, org, ps.key, (uint32_t)ps.depth);
+    if (!(ls.pdf_solid > 0.0f)) return;
+    Vec3 wi;
+    float tmax;
+    if (ls.kind == kLightArea) {
+        Vec3 to = ls.p - org;
+        float dist = length(to);
+        wi = to / dist;
+        tmax = dist * (1.0f - 1e-3f);
+    } else {
+        wi = ls.n;
+        tmax = kInf;
+    }
+    float cos_s = dot(f.n, wi);
+    if (cos_s > 0.0f) {
+        float pb = cos_s * kInvPi;
+        float w = ls.kind == kLightSun ? 1.0f : power_heuristic(ls.pdf_solid, pb);
+        *contrib = ps.throughput * m.albedo * ls.Le * (cos_s * kInvPi * w / ls.pdf_solid);
+        shadow->o = org;
+        shadow->d = wi;
+        shadow->tmin = kRayEps;
+        shadow->tmax = tmax;
+        *has_shadow = 1;
+    }
+}
+
 WF_HD bool shade_lambert(const SceneView& s, PathState& ps, const Surface& f, Ray* shadow,
                          Vec3* contrib, int* has_shadow) {
     const Material& m = s.materials[f.material];
     uint32_t d = (uint32_t)ps.depth;
-    *has_shadow = 0;
     Vec3 org = f.p + f.n * kRayEps;
-    if (s.n_sel > 0) {
-        LightSample ls = sample_light(s, org, ps.key, d);
-        if (ls.pdf_solid > 0.0f) {
-            Vec3 wi;
-            float tmax;
-            if (ls.kind == kLightArea) {
-                Vec3 to = ls.p - org;
-                float dist = length(to);
-                wi = to / dist;
-                tmax = dist * (1.0f - 1e-3f);
-            } else {
-                wi = ls.n;
-                tmax = kInf;
-            }
-            float cos_s = dot(f.n, wi);
-            if (cos_s > 0.0f) {
-                float pb = cos_s * kInvPi;
-                float w = ls.kind == kLightSun ? 1.0f : power_heuristic(ls.pdf_solid, pb);
-                *contrib = ps.throughput * m.albedo * ls.Le * (cos_s * kInvPi * w / ls.pdf_solid);
-                shadow->o = org;
-                shadow->d = wi;
-                shadow->tmin = kRayEps;
-                shadow->tmax = tmax;
-                *has_shadow = 1;
-            }
-        }
-    }
+    lambert_nee(s, ps, f, m, org, shadow, contrib, has_shadow);
     float u1 = rng(ps.key, d, kDimBsdfU), u2 = rng(ps.key, d, kDimBsdfV);
     float r = sqrtf(u1), phi = 2.0f * kPi * u2;
     float sp, cp;
@@ -826,6 +840,69 @@ WF_HD bool shade_dielectric(const SceneView& s, PathState& ps, const Surface& f)
     ps.prev_pdf = 0.0f;
     ps.depth += 1;
     set_next_ray(ps, org, normalize(dir));
+    return russian_roulette(ps);
+}
+
+// shade_lambert, shade_metal or shade_dielectric as `mat` (a type classify returned) says, in one
+// pass: each step that two materials share is written once, with its operands chosen per material
+// (the first scatter draw; the second draw, square root and sin/cos behind the Lambertian and the
+// metal-fuzz directions; the final normalisation, state update and roulette). A wave whose lanes
+// hold different materials so runs each such step once instead of once per material. Every lane
+// performs the operations of its own material's function, in the same order on the same values,
+// so the results are the same bits.
+WF_HD bool shade_surface(const SceneView& s, PathState& ps, const Surface& f, int mat, Ray* shadow, Vec3* contrib,
+                         int* has_shadow) {
+    const Material& m = s.materials[f.material];
+    const uint32_t d = (uint32_t)ps.depth;
+    const bool lam = mat == kLambert, met = mat == kMetal;   // otherwise dielectric
+    const Vec3 off = f.n * kRayEps;
+    Vec3 org = f.p + off;
+    *has_shadow = 0;
+    if (lam) lambert_nee(s, ps, f, m, org, shadow, contrib, has_shadow);
+    // Lambert's u1, the fuzz direction's u1, or the dielectric's reflect-or-refract draw.
+    const float u1 = rng(ps.key, d, lam ? kDimBsdfU : (met ? kDimFuzzU : kDimDielectric));
+    Vec3 raw;
+    if (lam || (met && m.fuzz > 0.0f)) {
+        // Lambert: r = sqrt(u1), z = sqrt(max(0, 1 - u1)); the fuzz direction (sphere_dir):
+        // z = 1 - 2 u1, r = sqrt(max(0, 1 - z z)); both at the angle 2 pi u2.
+        const float u2 = rng(ps.key, d, lam ? kDimBsdfV : kDimFuzzV);
+        const float zs = 1.0f - 2.0f * u1;
+        const float r = sqrtf(lam ? u1 : max2(0.0f, 1.0f - zs * zs));
+        const float phi = 2.0f * kPi * u2;
+        float sp, cp;
+        sincos_f32(phi, &sp, &cp);
+        if (lam) {
+            Vec3 t, b;
+            make_basis(f.n, &t, &b);
+            const float z = sqrtf(max2(0.0f, 1.0f - u1));
+            raw = t * (r * cp) + b * (r * sp) + f.n * z;
+        } else {
+            raw = reflect(ps.ray.d, f.n) + Vec3{r * cp, r * sp, zs} * m.fuzz;
+        }
+    } else if (met) {
+        raw = reflect(ps.ray.d, f.n);
+    } else {
+        const float eta_i = f.inside ? m.ior : 1.0f;
+        const float eta_t = f.inside ? 1.0f : m.ior;
+        const float cos_i = clampf(-dot(ps.ray.d, f.n), 0.0f, 1.0f);
+        const float F = fresnel_dielectric(cos_i, eta_i, eta_t);
+        if (u1 < F) {
+            raw = reflect(ps.ray.d, f.n);
+        } else {
+            const float eta = eta_i / eta_t;
+            const float sin2_t = eta * eta * (1.0f - cos_i * cos_i);
+            const float cos_t = sqrtf(max2(0.0f, 1.0f - sin2_t));
+            raw = ps.ray.d * eta + f.n * (eta * cos_i - cos_t);
+            org = f.p - off;
+        }
+    }
+    if (met && dot(raw, f.n) <= 0.0f) return false;   // scattered below the surface
+    const Vec3 dir = normalize(raw);
+    ps.throughput = ps.throughput * m.albedo;
+    ps.specular = lam ? 0 : 1;
+    ps.prev_pdf = lam ? dot(dir, f.n) * kInvPi : 0.0f;
+    ps.depth += 1;
+    set_next_ray(ps, org, dir);
     return russian_roulette(ps);
 }
 

@@ -584,7 +584,7 @@ __device__ __forceinline__ void shade_load(const PCtx& c, const PathBuf& in, con
 // the octant of their new direction (dir_bins = 8), each group in item order, so the waves of the
 // next extend see rays of nearby pixels heading the same way.
 template <bool SINGLE>
-__global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, PathBuf out) {
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) void pw_shade_kernel(PCtx c, PathBuf in, PathBuf out) {
     __shared__ uint32_t cnt[2][9][16];    // per round parity, per wave: 8 direction bins, shadow
     __shared__ uint32_t bases[2][2];
     __shared__ uint32_t pre[kSegs + 1], bas[kSegs], win[kSegs];
@@ -656,9 +656,7 @@ __global__ __launch_bounds__(1024) void pw_shade_kernel(PCtx c, PathBuf in, Path
             const Hit h{hv.x, __float_as_int(hv.y)};
             Surface f;
             const int mat = classify(s, ps, h, depth, &f);
-            if (mat == kLambert) live = shade_lambert(s, ps, f, &sh, &contrib, &has);
-            else if (mat == kMetal) live = shade_metal(s, ps, f);
-            else if (mat == kDielectric) live = shade_dielectric(s, ps, f);
+            if (mat >= kLambert && mat <= kDielectric) live = shade_surface(s, ps, f, mat, &sh, &contrib, &has);
             fin = !live && !has;
             if (!live) job.sbuf[sidx] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
         }

@@ -134,6 +134,65 @@ __global__ void trace_kernel(SceneView s, const int* pxys, int n, uint32_t seed,
     rays[2 * i + 1] = rc.shadow;
 }
 
+// One classify + shade step. A case is kShadeCaseWords words (ints as their bits): ray o xyz,
+// d xyz, hit t, hit primitive (device order, -1 for a miss), throughput xyz, L xyz, prev_pdf, depth,
+// specular, key k0, k1, max_depth. The output is kShadeOutWords words: alive, has_shadow, classify's
+// type, the path's ray (o xyz, d xyz, tmin, tmax), throughput xyz, L xyz, prev_pdf, depth, specular,
+// the shadow ray (o xyz, d xyz, tmin, tmax), its contribution xyz, then the hit primitive and t.
+// MERGED picks shade_surface over the three per-material functions.
+constexpr int kShadeCaseWords = 20, kShadeOutWords = 34;
+
+template <bool MERGED>
+__device__ __host__ inline void shade_case(const SceneView& s, const uint32_t* q, uint32_t* o) {
+    auto f = [&](int i) { return __builtin_bit_cast(float, q[i]); };
+    PathState ps;
+    ps.ray.o = Vec3{f(0), f(1), f(2)};
+    ps.ray.d = Vec3{f(3), f(4), f(5)};
+    ps.ray.tmin = kRayEps;
+    ps.ray.tmax = kInf;
+    const Hit h{f(6), (int)q[7]};
+    ps.throughput = Vec3{f(8), f(9), f(10)};
+    ps.L = Vec3{f(11), f(12), f(13)};
+    ps.prev_pdf = f(14);
+    ps.depth = (int)q[15];
+    ps.specular = (int)q[16];
+    ps.key = PathKey{q[17], q[18]};
+    Surface sf;
+    Ray sh{};
+    Vec3 c{0.0f, 0.0f, 0.0f};
+    int has = 0;
+    bool alive = false;
+    const int mat = classify(s, ps, h, (int)q[19], &sf);
+    if (MERGED) {
+        if (mat >= kLambert && mat <= kDielectric) alive = shade_surface(s, ps, sf, mat, &sh, &c, &has);
+    } else if (mat == kLambert) {
+        alive = shade_lambert(s, ps, sf, &sh, &c, &has);
+    } else if (mat == kMetal) {
+        alive = shade_metal(s, ps, sf);
+    } else if (mat == kDielectric) {
+        alive = shade_dielectric(s, ps, sf);
+    }
+    const float w[31] = {ps.ray.o.x,  ps.ray.o.y,  ps.ray.o.z,   ps.ray.d.x,   ps.ray.d.y,  ps.ray.d.z,  ps.ray.tmin,
+                         ps.ray.tmax, ps.throughput.x, ps.throughput.y, ps.throughput.z, ps.L.x, ps.L.y, ps.L.z,
+                         ps.prev_pdf, 0.0f,        0.0f,         sh.o.x,       sh.o.y,      sh.o.z,      sh.d.x,
+                         sh.d.y,      sh.d.z,      sh.tmin,      sh.tmax,      c.x,         c.y,         c.z,
+                         0.0f,        h.t,         0.0f};
+    o[0] = alive ? 1u : 0u;
+    o[1] = (uint32_t)has;
+    o[2] = (uint32_t)mat;
+    for (int i = 0; i < 31; ++i) o[3 + i] = __builtin_bit_cast(uint32_t, w[i]);
+    o[18] = (uint32_t)ps.depth;
+    o[19] = (uint32_t)ps.specular;
+    o[31] = (uint32_t)h.prim;
+    o[33] = 0u;
+}
+
+template <bool MERGED>
+__global__ void shade_case_kernel(SceneView s, const uint32_t* q, int n, uint32_t* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) shade_case<MERGED>(s, q + kShadeCaseWords * i, out + kShadeOutWords * i);
+}
+
 // Plain 16-byte-per-lane copy: the calibration kernel for hardware traffic counters (a kernel
 // whose device-memory bytes are known exactly: n16 * 16 read, n16 * 16 written).
 __global__ void copy_probe_kernel(const float4* __restrict__ src, float4* __restrict__ dst, size_t n16) {
@@ -350,6 +409,51 @@ int wf_test_trace_paths(const wf_scene* scene, const int* pxys, int n, uint32_t 
         check(hipDeviceSynchronize(), "trace sync");
         dl.get(L, (size_t)3 * n);
         dc.get(rays, (size_t)2 * n);
+    });
+}
+
+int wf_test_shade(const wf_scene* scene, const uint32_t* cases, int n, int form, uint32_t* out) {
+    return hook([&] {
+        if (!scene || n < 0 || (n > 0 && (!cases || !out))) throw std::invalid_argument("bad argument");
+        if (form < 0 || form > 3) throw std::invalid_argument("form must be 0..3");
+        const Scene& sc = scene->s;
+        const SceneView v = host_view(sc);
+        std::vector<uint32_t> q(cases, cases + (size_t)kShadeCaseWords * n);
+        for (int i = 0; i < n; ++i) {
+            uint32_t* c = q.data() + (size_t)kShadeCaseWords * i;
+            const int p = (int)c[7];
+            if (p == -2) {   // closest hit of the case's ray, found on the host
+                Ray r;
+                r.o = Vec3{__builtin_bit_cast(float, c[0]), __builtin_bit_cast(float, c[1]), __builtin_bit_cast(float, c[2])};
+                r.d = Vec3{__builtin_bit_cast(float, c[3]), __builtin_bit_cast(float, c[4]), __builtin_bit_cast(float, c[5])};
+                r.tmin = kRayEps;
+                r.tmax = kInf;
+                Hit h;
+                intersect(v, r, &h, false);
+                c[6] = __builtin_bit_cast(uint32_t, h.t);
+                c[7] = (uint32_t)h.prim;
+            } else if (p < -1 || p >= v.n_prims) {
+                throw std::invalid_argument("primitive index out of range");
+            }
+        }
+        if (form < 2) {
+            for (int i = 0; i < n; ++i) {
+                const uint32_t* c = q.data() + (size_t)kShadeCaseWords * i;
+                uint32_t* o = out + (size_t)kShadeOutWords * i;
+                if (form == 1) shade_case<true>(v, c, o);
+                else shade_case<false>(v, c, o);
+            }
+            return;
+        }
+        DeviceScene ds;
+        ds.upload(sc);
+        DevBuf<uint32_t> dq((size_t)kShadeCaseWords * n), dout((size_t)kShadeOutWords * n);
+        dq.put(q.data(), (size_t)kShadeCaseWords * n);
+        if (n && form == 3) shade_case_kernel<true><<<(n + 255) / 256, 256>>>(ds.view, dq.p, n, dout.p);
+        if (n && form == 2) shade_case_kernel<false><<<(n + 255) / 256, 256>>>(ds.view, dq.p, n, dout.p);
+        check(hipGetLastError(), "shade kernel");
+        check(hipDeviceSynchronize(), "shade sync");
+        dout.get(out, (size_t)kShadeOutWords * n);
     });
 }
 

@@ -127,7 +127,8 @@ EXPORTS = [
     "wf_test_queue_push",
     "wf_batch_render", "wf_autotune", "wf_autotune_budget", "wf_server_create", "wf_server_destroy", "wf_server_submit",
     "wf_server_step", "wf_server_poll", "wf_server_stats_get", "wf_test_sincos", "wf_test_sincos_grid_hash",
-    "wf_test_intersect", "wf_test_slab", "wf_test_divu32", "wf_test_lds_plan", "wf_test_trace_paths", "wf_copy_probe",
+    "wf_test_intersect", "wf_test_slab", "wf_test_divu32", "wf_test_lds_plan", "wf_test_trace_paths", "wf_test_shade",
+    "wf_copy_probe",
     "wf_cached_memory_bytes", "wf_release_cached_memory",
 ]
 
@@ -185,6 +186,7 @@ def lib():
         "wf_test_lds_plan": (I, [C.POINTER(C.c_int), C.POINTER(C.c_int), I, C.c_int64, C.POINTER(C.c_int),
                                  C.POINTER(C.c_int)]),
         "wf_test_trace_paths": (I, [P, C.POINTER(C.c_int), I, C.c_uint32, I, F, C.POINTER(C.c_uint32), I]),
+        "wf_test_shade": (I, [P, C.POINTER(C.c_uint32), I, I, C.POINTER(C.c_uint32)]),
         "wf_cached_memory_bytes": (U64, []), "wf_release_cached_memory": (I, [C.POINTER(C.c_uint64)]),
     }
     for name, (res, args) in sig.items():
@@ -478,6 +480,20 @@ def test_trace_paths(scene, pxys, seed=1234, max_depth=8, device=False):
     _check(lib().wf_test_trace_paths(scene.handle, _iptr(q), len(q), seed, max_depth, _fptr(L),
                                      rays.ctypes.data_as(C.POINTER(C.c_uint32)), int(device)))
     return L, rays
+
+
+SHADE_CASE_WORDS, SHADE_OUT_WORDS = 20, 34
+
+
+def test_shade(scene, cases, form=0):
+    """One classify + shade step per case (SHADE_CASE_WORDS uint32 words, floats as their bits).
+    form: 0 the per-material functions on the host, 1 the one-pass shade_surface on the host, 2 and 3
+    the same on the GPU. A case whose primitive word is -2 has its closest hit found on the host."""
+    q = np.ascontiguousarray(cases, np.uint32).reshape(-1, SHADE_CASE_WORDS)
+    out = np.zeros((len(q), SHADE_OUT_WORDS), np.uint32)
+    u32 = C.POINTER(C.c_uint32)
+    _check(lib().wf_test_shade(scene.handle, q.ctypes.data_as(u32), len(q), int(form), out.ctypes.data_as(u32)))
+    return out
 
 
 def make_job(scene, spp, seed=1234, max_depth=8, width=0, height=0, yaw_deg=None):

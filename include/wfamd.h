@@ -228,6 +228,15 @@ int wf_test_lds_plan(const int* n_nodes, const int* n_prims, int n, int64_t budg
  * counts (extension, shadow). */
 int wf_test_trace_paths(const wf_scene* scene, const int* pxys, int n, uint32_t seed, int max_depth, float* L,
                         uint32_t* rays, int device);
+/* One classify + shade step (include/wf/path_core.hpp:151-276) per case of 20 words (floats and
+ * ints as their bits): ray o xyz, d xyz, hit t, hit primitive (device order; -1 a miss; -2: the
+ * closest hit is found on the host), throughput xyz, L xyz, prev_pdf, depth, specular, key k0, k1,
+ * max_depth. form 0: the per-material shading functions on the host, 1: the one-pass form the
+ * persistent engine runs (no reference counterpart), 2 and 3: the same two on the GPU. out: 34 words
+ * per case: alive, has_shadow, material type (-1 ended), ray o xyz, d xyz, tmin, tmax, throughput
+ * xyz, L xyz, prev_pdf, depth, specular, shadow ray o xyz, d xyz, tmin, tmax, its contribution xyz,
+ * hit primitive, hit t, 0. */
+int wf_test_shade(const wf_scene* scene, const uint32_t* cases, int n, int form, uint32_t* out);
 
 /* ---- memory (no reference counterpart) ---- */
 /* Persistent engines (renderers, batches, the serving loop) hand their large device blocks (path
