@@ -173,10 +173,16 @@ WF_HD uint32_t path_key_k1(uint32_t b, uint32_t pixel, uint32_t sample) {
     return hash_u32(hash_u32(b ^ (pixel * 0x9e3779b1U)) + (sample * 0x85ebca77U + 0xc2b2ae3dU));
 }
 
-WF_HD PathKey make_path_key(uint32_t seed, uint32_t pixel, uint32_t sample) {
-    uint32_t a = hash_u32(seed ^ 0x9e3779b9U);
+// The key from the seed's two hashes a = path_key_a(seed) and b = path_key_b(seed), which a batched
+// engine computes once per job.
+WF_HD uint32_t path_key_a(uint32_t seed) { return hash_u32(seed ^ 0x9e3779b9U); }
+WF_HD PathKey path_key_ab(uint32_t a, uint32_t b, uint32_t pixel, uint32_t sample) {
     uint32_t k0 = hash_u32(hash_u32(a + pixel * 0xc2b2ae35U) ^ (sample * 0x27d4eb2fU + 0x165667b1U));
-    return PathKey{k0, path_key_k1(path_key_b(seed), pixel, sample)};
+    return PathKey{k0, path_key_k1(b, pixel, sample)};
+}
+
+WF_HD PathKey make_path_key(uint32_t seed, uint32_t pixel, uint32_t sample) {
+    return path_key_ab(path_key_a(seed), path_key_b(seed), pixel, sample);
 }
 
 // Dimension slots per depth (< 16). 0-8 are the reference's (include/wf/rng.hpp:30-35);
@@ -275,6 +281,7 @@ struct SceneView {
     int n_nodes, n_prims, n_spheres, n_materials;
     int n_lights;   // emissive primitives
     int n_sel;      // lights in the selection pool: n_lights + sky_light + sun_light
+    float sky_pdf;  // kInv4Pi / n_sel: the sky's solid-angle pdf with its selection probability
     EnvParams env;
     CameraRig camera;
     int width, height;
@@ -585,10 +592,9 @@ WF_HD Ray camera_ray(const SceneView& s, int px, int py, PathKey key) {
     return camera_ray_view(s.camera, s.width, s.height, px, py, key);
 }
 
-WF_HD PathState init_path_view(const CameraRig& cam, int width, int height, int px, int py, uint32_t sample,
-                               uint32_t seed) {
+WF_HD PathState init_path_keyed(const CameraRig& cam, int width, int height, int px, int py, PathKey key) {
     PathState ps;
-    ps.key = make_path_key(seed, (uint32_t)(py * width + px), sample);
+    ps.key = key;
     ps.ray = camera_ray_view(cam, width, height, px, py, ps.key);
     ps.throughput = Vec3{1.0f, 1.0f, 1.0f};
     ps.L = Vec3{0.0f, 0.0f, 0.0f};
@@ -596,6 +602,11 @@ WF_HD PathState init_path_view(const CameraRig& cam, int width, int height, int 
     ps.specular = 1;
     ps.prev_pdf = 0.0f;
     return ps;
+}
+
+WF_HD PathState init_path_view(const CameraRig& cam, int width, int height, int px, int py, uint32_t sample,
+                               uint32_t seed) {
+    return init_path_keyed(cam, width, height, px, py, make_path_key(seed, (uint32_t)(py * width + px), sample));
 }
 
 WF_HD PathState init_path(const SceneView& s, int px, int py, uint32_t sample, uint32_t seed) {
@@ -639,7 +650,8 @@ WF_HD LightSample sample_light(const SceneView& s, Vec3 from, PathKey key, uint3
     ls.Le = Vec3{0, 0, 0};
     ls.kind = kLightArea;
     if (s.n_sel <= 0) return ls;
-    int idx = (int)(rng(key, depth, kDimLightPick) * (float)s.n_sel);
+    // With one light the pick is 0 for every draw in [0, 1), so the draw is skipped.
+    int idx = s.n_sel == 1 ? 0 : (int)(rng(key, depth, kDimLightPick) * (float)s.n_sel);
     if (idx >= s.n_sel) idx = s.n_sel - 1;
     float u1 = rng(key, depth, kDimLightU), u2 = rng(key, depth, kDimLightV);
     if (idx >= s.n_lights) {
@@ -647,7 +659,7 @@ WF_HD LightSample sample_light(const SceneView& s, Vec3 from, PathKey key, uint3
             ls.kind = kLightSky;
             ls.n = sphere_dir(u1, u2);
             ls.Le = sky_radiance(s.env, ls.n);
-            ls.pdf_solid = kInv4Pi / (float)s.n_sel;
+            ls.pdf_solid = s.sky_pdf;
         } else {
             ls.kind = kLightSun;
             ls.n = s.env.sun_dir;
@@ -693,7 +705,7 @@ WF_HD int classify(const SceneView& s, PathState& ps, const Hit& h, int max_dept
     if (h.prim < 0) {
         if (s.env.sky_light && !ps.specular && ps.prev_pdf > 0.0f) {
             Vec3 Le = sky_radiance(s.env, ps.ray.d);
-            float w = power_heuristic(ps.prev_pdf, kInv4Pi / (float)s.n_sel);
+            float w = power_heuristic(ps.prev_pdf, s.sky_pdf);
             add_to(ps.L, ps.throughput * Le * w);
         } else {
             add_to(ps.L, ps.throughput * sky_radiance(s.env, ps.ray.d));

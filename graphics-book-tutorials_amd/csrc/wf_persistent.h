@@ -84,7 +84,7 @@ struct JobDev {
     int scene;
     int width, height, max_depth;
     CameraRig cam;
-    uint32_t first_sample, spp, seed, key_b;   // key_b = path_key_b(seed)
+    uint32_t first_sample, spp, key_a, key_b;   // path_key_a(seed), path_key_b(seed)
     DivU32 div_pixels, div_band;   // by width * height and by 8 * width (new-path indexing)
     unsigned long long work_begin, work_end;
     float4* sbuf;

@@ -161,7 +161,8 @@ __device__ __forceinline__ PathState path_of_job(const JobDev& job, uint32_t jid
     tile_pixel(local - q * WH, W, (uint32_t)job.height, job.div_band, &px, &py);
     *jid_out = jid;
     *sidx_out = q * WH + py * W + px;
-    return init_path_view(job.cam, job.width, job.height, (int)px, (int)py, job.first_sample + q, job.seed);
+    return init_path_keyed(job.cam, job.width, job.height, (int)px, (int)py,
+                           path_key_ab(job.key_a, job.key_b, py * W + px, job.first_sample + q));
 }
 
 typedef __attribute__((address_space(4))) const JobDev const_job;
@@ -181,7 +182,8 @@ __device__ __forceinline__ JobDev job_fields_scalar(const JobDev* jobs, uint32_t
     d.cam.horizontal = Vec3{p->cam.horizontal.x, p->cam.horizontal.y, p->cam.horizontal.z};
     d.cam.vertical = Vec3{p->cam.vertical.x, p->cam.vertical.y, p->cam.vertical.z};
     d.first_sample = p->first_sample;
-    d.seed = p->seed;
+    d.key_a = p->key_a;
+    d.key_b = p->key_b;
     d.scene = p->scene;
     return d;
 }
@@ -1344,7 +1346,7 @@ JobDev PersistentEngine::prepare_job(const PersistentJob& job, float4* sbuf, int
     d.max_depth = job.max_depth;
     d.first_sample = (uint32_t)job.first_sample;
     d.spp = (uint32_t)job.spp;
-    d.seed = job.seed;
+    d.key_a = path_key_a(job.seed);
     d.key_b = path_key_b(job.seed);
     d.work_begin = m.total_work;
     d.work_end = m.total_work + work;

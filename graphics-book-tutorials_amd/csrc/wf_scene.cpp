@@ -394,6 +394,7 @@ SceneView host_view(const Scene& s) {
     v.n_materials = (int)s.materials.size();
     v.n_lights = (int)s.lights.size();
     v.n_sel = v.n_lights + (s.env.sky_light ? 1 : 0) + (s.env.sun_light ? 1 : 0);
+    v.sky_pdf = v.n_sel > 0 ? kInv4Pi / (float)v.n_sel : 0.0f;
     v.env = s.env;
     v.camera = make_camera_rig(s.camera, s.width, s.height);
     v.width = s.width;
