@@ -246,6 +246,9 @@ __device__ __forceinline__ void trav_visit_signed(const Src& src, Traversal& tr,
 // round; what changes is how often the wave stops to count its parked and moving lanes (two
 // ballots and a branch per round). Three visits took sphere_grid 1280x720x64 from 13.19 to
 // 12.33 ms per render (extend 7.59 -> 7.02 ms, shadow 1.79 -> 1.48 ms); two gave 12.42, four 12.34.
+// The first visit stays outside the loop: folded into it (`k == 0 ||`), the single-job shadow
+// kernel took 65 VGPRs instead of 63, which leaves room for only one 1024-thread workgroup per CU
+// (4 waves per SIMD instead of 8) and cost 0.35 ms per render.
 constexpr int kVisitsPerRound = 3;
 
 // MODE 0: trav_visit (any ray); 1: trav_visit_signed; 2: trav_visit_signed with every node in LDS.
@@ -263,9 +266,11 @@ __device__ __forceinline__ void wave_traverse_loop(int n_nodes, const Src& src, 
         } else if (moving) {
             // Up to kVisitsPerRound visits before the wave counts its lanes again: a lane that
             // parks a leaf or finishes stops; the others go on.
+            if constexpr (MODE == 0) trav_visit(n_nodes, src, tr, leaf);
+            else trav_visit_signed<MODE == 2>(src, tr, leaf, neg_x, neg_y, neg_z);
 #pragma unroll
-            for (int k = 0; k < kVisitsPerRound; ++k) {
-                if (k == 0 || (tr.node >= 0 && !leaf_parked(leaf))) {
+            for (int k = 1; k < kVisitsPerRound; ++k) {
+                if (tr.node >= 0 && !leaf_parked(leaf)) {
                     if constexpr (MODE == 0) trav_visit(n_nodes, src, tr, leaf);
                     else trav_visit_signed<MODE == 2>(src, tr, leaf, neg_x, neg_y, neg_z);
                 }
