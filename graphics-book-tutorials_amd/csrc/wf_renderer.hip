@@ -5,6 +5,8 @@
 // src/gpu/gpu_renderer.cu:233-237).
 #include <algorithm>
 #include <cstring>
+#include <exception>
+#include <thread>
 
 #include "wf_megakernel.h"
 #include "wf_persistent.h"
@@ -266,12 +268,32 @@ void GpuRenderer::download_to(int spp_done, float* rgb, float* lum_var) {
         }
         gpu::check(hipEventRecord(m.dl_ev[k], m.stream), "hipEventRecord");
     }
-    for (int k = 0; k < K; ++k) {
+    // Each range is finalized on its own host thread as soon as its copy has landed (range 0 on
+    // this one): the binary64 finalize of a 720p image takes longer than its copy.
+    auto finish = [&](int k) {
         gpu::check(hipEventSynchronize(m.dl_ev[k]), "download sync");
         const size_t a = std::min(P, k * step), n = std::min(step, P - a);
         finalize_film(m.pinned + 3 * a, m.pinned + 3 * P + a, m.pinned + 4 * P + a, (int)n, spp_done,
                       rgb ? rgb + 3 * a : nullptr, lum_var ? lum_var + a : nullptr);
+    };
+    std::exception_ptr err[4];
+    std::vector<std::thread> helpers;
+    for (int k = 1; k < K; ++k)
+        helpers.emplace_back([&, k] {
+            try {
+                finish(k);
+            } catch (...) {
+                err[k] = std::current_exception();
+            }
+        });
+    try {
+        finish(0);
+    } catch (...) {
+        err[0] = std::current_exception();
     }
+    for (std::thread& t : helpers) t.join();
+    for (int k = 0; k < K; ++k)
+        if (err[k]) std::rethrow_exception(err[k]);
 }
 
 void GpuRenderer::download_mean(int spp_done, std::vector<float>* rgb) {
