@@ -11,6 +11,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/wfamd.h"
@@ -160,6 +161,23 @@ struct PinnedStage {
     }
 };
 
+// Finalizes the n pixels of a staged chunk (rgb sums, then luminance sums, then squared sums) on up
+// to four host threads, the calling one included: the binary64 variance of a chunk takes longer
+// than its copy.
+static void finalize_split(const float* h, size_t n, int spp, float* rgb, float* lum_var) {
+    const size_t parts = n >= 65536 ? 4 : 1, step = (n + parts - 1) / parts;
+    auto part = [&](size_t k) {
+        const size_t b = std::min(n, k * step), m = std::min(step, n - b);
+        if (m)
+            wf::finalize_film(h + 3 * b, h + 3 * n + b, h + 4 * n + b, (int)m, spp, rgb ? rgb + 3 * b : nullptr,
+                              lum_var ? lum_var + b : nullptr);
+    };
+    std::vector<std::thread> helpers;
+    for (size_t k = 1; k < parts; ++k) helpers.emplace_back(part, k);
+    part(0);
+    for (std::thread& t : helpers) t.join();
+}
+
 struct DevFilmAlloc {
     wf::gpu::DeviceFilm film{};
     int pixels = 0;
@@ -202,8 +220,7 @@ struct DevFilmAlloc {
             wf::gpu::check(hipEventSynchronize(stage.ev[s]), "download sync");
             const float* h = base + s * half;
             const size_t n = std::min(chunk, P - a);
-            wf::finalize_film(h, h + 3 * n, h + 4 * n, (int)n, spp, rgb ? rgb + 3 * a : nullptr,
-                              lum_var ? lum_var + a : nullptr);
+            finalize_split(h, n, spp, rgb ? rgb + 3 * a : nullptr, lum_var ? lum_var + a : nullptr);
         }
     }
     void release() {
